@@ -1,0 +1,266 @@
+"""empi_amd: Matching Pursuit decomposition on AMD MI355X (gfx950).
+
+Python view of the C ABI in include/empi_hip.h (libempi_hip.so, built in
+tree by `python -m empi_amd.build`).  The library is required: there is no
+CPU fallback, and every entry point raises if the HIP engine is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libempi_hip.so")
+
+SMP, MMP1, MMP2, MMP3 = 0, 1, 2, 3
+VARIANTS = {"smp": SMP, "mmp1": MMP1, "mmp2": MMP2, "mmp3": MMP3}
+
+OK = 0
+STATUS_NAMES = {
+    0: "ok", 1: "signalIsEmpty", 2: "energyMapIsEmpty",
+    3: "signalFileIsTooLongForThisDecomposition", 4: "invalidArgument",
+    5: "insufficientMemory", 6: "internalError", 7: "deviceError", 8: "unsupportedSize",
+}
+
+EXPORTS = [
+    "empi_ws_create", "empi_ws_compute", "empi_ws_find_best", "empi_ws_atom_count",
+    "empi_ws_subtract", "empi_ws_destroy", "empi_subtract_atom_from_signal",
+    "empi_plan_create", "empi_plan_run", "empi_plan_item_result", "empi_plan_item_atom_count",
+    "empi_plan_entries_evaluated", "empi_plan_set_profiling", "empi_plan_kernel_classes",
+    "empi_plan_kernel_stats", "empi_plan_destroy", "empi_strerror", "empi_version",
+]
+
+
+class DictParams(C.Structure):
+    _fields_ = [("energyError", C.c_double), ("scaleMin", C.c_double),
+                ("scaleMax", C.c_double), ("freqMax", C.c_double)]
+
+
+class Atom(C.Structure):
+    _fields_ = [("params", C.c_double * 6), ("scale", C.c_int), ("freq", C.c_int),
+                ("time", C.c_int), ("reserved", C.c_int)]
+
+
+class Item(C.Structure):
+    _fields_ = [("channels", C.c_int), ("samples", C.c_int), ("fs", C.c_double),
+                ("variant", C.c_int), ("iterationMax", C.c_int), ("energyPercent", C.c_double),
+                ("dict", DictParams)]
+
+
+class EmpiError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = status
+        name = STATUS_NAMES.get(status, "internalError")
+        super().__init__(f"{name}" + (f" ({where})" if where else ""))
+
+
+_lib = None
+
+
+def lib():
+    """Loads libempi_hip.so; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP engine not built: {LIB_PATH} (python -m empi_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.empi_ws_create.argtypes = [C.POINTER(DictParams), C.c_double, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(vp)]
+        L.empi_ws_compute.argtypes = [vp, dp]
+        L.empi_ws_find_best.argtypes = [vp, C.POINTER(Atom)]
+        L.empi_ws_atom_count.argtypes = [vp]
+        L.empi_ws_atom_count.restype = C.c_size_t
+        L.empi_ws_subtract.argtypes = [vp, C.POINTER(Atom), C.c_int, dp]
+        L.empi_ws_destroy.argtypes = [vp]
+        L.empi_ws_destroy.restype = None
+        L.empi_subtract_atom_from_signal.argtypes = [C.c_double, C.POINTER(Atom), dp, C.c_int, C.c_int]
+        L.empi_plan_create.argtypes = [C.POINTER(Item), C.c_int, C.POINTER(vp)]
+        L.empi_plan_run.argtypes = [vp, vp, C.c_int, vp]
+        L.empi_plan_item_result.argtypes = [vp, C.c_int, C.POINTER(Atom), C.POINTER(C.c_int), dp]
+        L.empi_plan_item_atom_count.argtypes = [vp, C.c_int]
+        L.empi_plan_item_atom_count.restype = C.c_size_t
+        L.empi_plan_entries_evaluated.argtypes = [vp]
+        L.empi_plan_entries_evaluated.restype = C.c_double
+        L.empi_plan_set_profiling.argtypes = [vp, C.c_int]
+        L.empi_plan_kernel_classes.restype = C.c_int
+        L.empi_plan_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong),
+                                             dp, dp, dp]
+        L.empi_plan_destroy.argtypes = [vp]
+        L.empi_plan_destroy.restype = None
+        L.empi_strerror.argtypes = [C.c_int]
+        L.empi_strerror.restype = C.c_char_p
+        L.empi_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(status, where=""):
+    if status != OK:
+        raise EmpiError(status, where)
+
+
+def dict_params(energy_error, scale_min=0.0, scale_max=math.inf, freq_max=math.inf):
+    return DictParams(float(energy_error), float(scale_min), float(scale_max), float(freq_max))
+
+
+def _atoms_to_arrays(atoms, shape):
+    params = np.array([list(a.params) for a in atoms], dtype=np.float64).reshape(shape + (6,))
+    index = np.array([(a.scale, a.freq, a.time) for a in atoms], dtype=np.int64).reshape(shape + (3,))
+    return params, index
+
+
+@dataclass
+class ItemResult:
+    status: int
+    params: np.ndarray      # [C][I][6]
+    index: np.ndarray       # [C][I][3] (scale, bin, time)
+    counts: np.ndarray      # [C]
+    energies: np.ndarray    # [rows][I+1]
+
+
+@dataclass
+class ItemSpec:
+    channels: int
+    samples: int
+    fs: float
+    variant: int
+    iteration_max: int
+    energy_percent: float
+    energy_error: float
+    scale_min: float = 0.0
+    scale_max: float = math.inf
+    freq_max: float = math.inf
+
+    def to_c(self):
+        return Item(self.channels, self.samples, float(self.fs), int(self.variant),
+                    int(self.iteration_max), float(self.energy_percent),
+                    dict_params(self.energy_error, self.scale_min, self.scale_max, self.freq_max))
+
+
+class Plan:
+    """Batched device-resident decomposition (empi_plan_*)."""
+
+    def __init__(self, items):
+        self.items = list(items)
+        arr = (Item * len(self.items))(*[it.to_c() for it in self.items])
+        h = C.c_void_p()
+        _check(lib().empi_plan_create(arr, len(self.items), C.byref(h)), "empi_plan_create")
+        self._h = h
+        self.total_samples = sum(it.channels * it.samples for it in self.items)
+
+    def run(self, signal=None, device_ptr=None, stream=None):
+        """signal: float32 host array with every item's [C][N] samples in
+        order; or device_ptr: integer device address of the same layout."""
+        st_ptr = C.c_void_p(stream) if stream else None
+        if device_ptr is not None:
+            status = lib().empi_plan_run(self._h, C.c_void_p(int(device_ptr)), 1, st_ptr)
+        else:
+            buf = np.ascontiguousarray(signal, dtype=np.float32).ravel()
+            if buf.size != self.total_samples:
+                raise ValueError(f"expected {self.total_samples} samples, got {buf.size}")
+            status = lib().empi_plan_run(self._h, buf.ctypes.data_as(C.c_void_p), 0, st_ptr)
+        _check(status, "empi_plan_run")
+
+    def result(self, i) -> ItemResult:
+        it = self.items[i]
+        Cn, I = it.channels, it.iteration_max
+        rows = Cn if it.variant == SMP else 1
+        atoms = (Atom * (Cn * I))()
+        counts = np.zeros(Cn, dtype=np.int32)
+        energies = np.zeros(rows * (I + 1))
+        status = lib().empi_plan_item_result(self._h, i, atoms,
+                                             counts.ctypes.data_as(C.POINTER(C.c_int)),
+                                             energies.ctypes.data_as(C.POINTER(C.c_double)))
+        params, index = _atoms_to_arrays(atoms, (Cn, I))
+        return ItemResult(status, params, index, counts, energies.reshape(rows, I + 1))
+
+    def atom_count(self, i):
+        return lib().empi_plan_item_atom_count(self._h, i)
+
+    def entries_evaluated(self):
+        return lib().empi_plan_entries_evaluated(self._h)
+
+    def set_profiling(self, on=True):
+        _check(lib().empi_plan_set_profiling(self._h, int(on)), "empi_plan_set_profiling")
+
+    def kernel_stats(self):
+        """Per sub-transform class of the last run: dicts with sub_length,
+        launches, total_ms, entries, entry_channels (classes never launched
+        are omitted)."""
+        out = []
+        for c in range(lib().empi_plan_kernel_classes()):
+            q = C.c_int()
+            n = C.c_longlong()
+            ms, e, ec = C.c_double(), C.c_double(), C.c_double()
+            _check(lib().empi_plan_kernel_stats(self._h, c, C.byref(q), C.byref(n), C.byref(ms),
+                                                C.byref(e), C.byref(ec)), "empi_plan_kernel_stats")
+            if n.value or e.value:
+                out.append({"class": c, "sub_length": q.value, "launches": n.value,
+                            "total_ms": ms.value, "entries": e.value, "entry_channels": ec.value})
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().empi_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Workspace:
+    """Per-call Workspace operator API (empi_ws_*), reference granularity."""
+
+    def __init__(self, fs, channels, samples, energy_error, scale_min=0.0, scale_max=math.inf,
+                 freq_max=math.inf, constraint=0):
+        h = C.c_void_p()
+        dp = dict_params(energy_error, scale_min, scale_max, freq_max)
+        _check(lib().empi_ws_create(C.byref(dp), float(fs), int(channels), int(samples),
+                                    int(constraint), C.byref(h)), "empi_ws_create")
+        self._h = h
+        self.fs, self.channels, self.samples = float(fs), int(channels), int(samples)
+
+    def compute(self, signal):
+        x = np.ascontiguousarray(np.atleast_2d(signal), dtype=np.float64)
+        assert x.shape == (self.channels, self.samples)
+        _check(lib().empi_ws_compute(self._h, x.ctypes.data_as(C.POINTER(C.c_double))),
+               "empi_ws_compute")
+
+    def find_best(self):
+        out = (Atom * self.channels)()
+        _check(lib().empi_ws_find_best(self._h, out), "empi_ws_find_best")
+        return list(out)
+
+    def atom_count(self):
+        return lib().empi_ws_atom_count(self._h)
+
+    def subtract(self, atom, channel, residual):
+        """Subtracts atom from residual (float64 numpy array, updated in place)."""
+        assert residual.dtype == np.float64 and residual.flags.c_contiguous
+        _check(lib().empi_ws_subtract(self._h, C.byref(atom), int(channel),
+                                      residual.ctypes.data_as(C.POINTER(C.c_double))),
+               "empi_ws_subtract")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().empi_ws_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def version():
+    return lib().empi_version().decode()

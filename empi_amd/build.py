@@ -1,0 +1,62 @@
+"""In-tree build of the HIP engine (gfx950) and the C++ host tools.
+
+    python -m empi_amd.build          # libempi_hip.so + empi-hip CLI
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 (v_pk_*_f32) stays off on this part
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "csrc"),
+]
+
+ENGINE_SOURCES = ["csrc/engine.hip", "csrc/dictionary.cpp"]
+LIB = os.path.join(HERE, "libempi_hip.so")
+
+HOST_SOURCES = ["host/empi_main.cpp", "host/workspace.cpp", "host/config.cpp", "host/book.cpp",
+                "host/signal_reader.cpp"]
+CLI = os.path.join(HERE, "empi-hip")
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_engine():
+    srcs = [os.path.join(HERE, s) for s in ENGINE_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_host():
+    srcs = [os.path.join(HERE, s) for s in HOST_SOURCES if os.path.exists(os.path.join(HERE, s))]
+    if len(srcs) != len(HOST_SOURCES):
+        return None
+    tmp = CLI + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+          "-I" + os.path.join(HERE, "host"), "-o", tmp] + srcs +
+         ["-L" + HERE, "-lempi_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI)
+    return CLI
+
+
+def build_all():
+    build_engine()
+    build_host()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

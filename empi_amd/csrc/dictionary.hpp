@@ -1,0 +1,58 @@
+// Optimal Gabor dictionary, host side (FP64).
+//
+// Restates GaborWorkspaceBuilder::prepareWorkspace (src/gabor.cpp:213-252)
+// expression by expression, so scales, FFT sizes and grid sizes are
+// bit-identical to the reference's: lrint is round-half-even under the
+// default rounding mode, exactly as the reference calls it.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "empi_hip.h"
+
+namespace empi {
+
+constexpr double kGaussHalfWidth = 3.0;        // src/gabor.cpp:18
+constexpr double kMinScaleInSamples = 2.0;     // src/gabor.cpp:23
+
+struct Scale {
+    double s;        // scale [s]
+    int Ngauss;
+    int Nfft;        // power of two
+    int T;           // time grid size
+    int F;           // frequency grid size
+    // launch geometry of the fused column kernels
+    int Wmax;        // longest window (samples) over the time grid
+    int Q;           // sub-transform length (complex points)
+    int P;           // Nfft/2 / Q sub-transforms per column
+    int G;           // cells per column = max(1, P/2)
+};
+
+struct Dictionary {
+    double fs = 0, tMax = 0, root = 0;
+    int N = 0;
+    empi_dict_params params{};
+    std::vector<Scale> scales;
+    long long entries = 0;       // sum F*T
+    long long cells = 0;         // sum T*G
+};
+
+// Window [iL, iR] of column t of scale s (src/gabor.cpp:284-286).
+inline void column_window(double fs, int N, double tMax, int T, double s, int t, int* iL, int* iR,
+                          double* t0fixed) {
+    const long n = N;
+    const double t0 = t * tMax / (T - 1);
+    const double hw = kGaussHalfWidth * s;
+    long l = std::lrint((t0 - hw) * fs);
+    long r = std::lrint((t0 + hw) * fs);
+    *iL = static_cast<int>(l > 0 ? l : 0);
+    *iR = static_cast<int>(r < n - 1 ? r : n - 1);
+    *t0fixed = t0 - *iL / fs;
+}
+
+// Builds the dictionary; returns EMPI_OK, EMPI_INVALID_ARGUMENT or EMPI_TOO_LONG.
+empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictionary* out);
+
+}  // namespace empi

@@ -1,0 +1,1538 @@
+// MI355X (gfx950) Matching Pursuit engine: kernels, batched plan and C ABI.
+//
+// Hot path (one MP iteration of every unit of a batch):
+//   k_select    one workgroup per unit: argmax over the unit's cell bests,
+//               FP64 refit of the chosen column, atom parameters, FP64
+//               residual update and residual energy / stop test.
+//   k_columns<Q> for every FFT-size class Q: re-evaluates the map cells the
+//               last atom touched -- window, Gaussian, packed real FFT in LDS
+//               (Stockham radix 8/4/2), phase twiddle, product, per-cell max.
+//               The complex map is never stored: a cell best is 8 bytes.
+// References (paths in the reference tree): gabor.cpp:75-133,143-209,
+// 279-354; classes.cpp:26-87; mmp.cpp:9-45.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <tuple>
+#include <vector>
+
+#include "dictionary.hpp"
+#include "empi_hip.h"
+#include "kernels.hpp"
+
+using namespace empi;
+
+#define EMPI_HIP(call)                                  \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) {                         \
+            std::fprintf(stderr, "empi_hip: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(e_), \
+                         __FILE__, __LINE__);           \
+            return EMPI_DEVICE;                         \
+        }                                               \
+    } while (0)
+
+namespace {
+
+constexpr int kMaxQ = 8192;
+constexpr int kMinLgQ = 1;    // Q = 2
+constexpr int kMaxLgQ = 13;   // Q = 8192
+constexpr int kSelectThreads = 256;
+
+// ------------------------------------------------------------ device math
+
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i
+__device__ inline float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// forward DFTs, w = e^{-2 pi i / R}
+__device__ inline void dft2(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+__device__ inline void dft4(float2* v) {
+    float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    float2 s13 = cadd(v[1], v[3]), d13 = cmul_mi(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+}
+__device__ inline void dft8(float2* v) {
+    const float r = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft4(e);
+    dft4(o);
+    // o_k *= e^{-i pi k / 4}
+    float2 o1 = make_float2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
+    float2 o2 = cmul_mi(o[2]);
+    float2 o3 = make_float2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+template <int R>
+__device__ inline void dft(float2* v) {
+    if constexpr (R == 2) dft2(v);
+    else if constexpr (R == 4) dft4(v);
+    else dft8(v);
+}
+
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
+// Workgroup geometry of class Q: NSL slots of Q complex points in LDS,
+// 16 points per thread.
+template <int Q>
+struct Geo {
+    static constexpr int NSL = Q >= 2048 ? 2 : 4096 / Q;
+    static constexpr int NPTS = NSL * Q;
+    static constexpr int NT = NPTS / 16;
+    static constexpr int NCELL = NSL / 2;
+    static constexpr int EPT = 16;
+};
+
+// LDS index with one float2 of padding per 8: the stride-8 writes of the
+// first radix-8 passes then spread over all banks.
+__device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> 3); }
+template <int NPTS>
+struct Lds {
+    static constexpr int SIZE = NPTS + NPTS / 8;
+};
+
+// One Stockham autosort pass of radix R at sub-length Ns over all slots.
+template <int Q, int NT, int NPTS, int Ns, int R>
+__device__ inline void stockham_pass(float2* buf, const float2* __restrict__ tw, int lgTWN) {
+    constexpr int BPS = Q / R;
+    constexpr int NB = NPTS / R;
+    constexpr int PER = NB / NT;
+    static_assert(PER * NT == NB, "pass does not tile the workgroup");
+    const int tid = threadIdx.x;
+    float2 v[PER][R];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int bi = tid + b * NT;
+        const int slot = bi / BPS, j = bi % BPS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = buf[lpad(slot * Q + j + r * BPS)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int bi = tid + b * NT;
+        const int slot = bi / BPS, j = bi % BPS;
+        const int k = j & (Ns - 1);
+        if constexpr (Ns > 1) {
+            const int sh = lgTWN - ilog2c(Ns * R);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[(r * k) << sh]);
+        }
+        dft<R>(v[b]);
+        const int base = slot * Q + (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[lpad(base + r * Ns)] = v[b][r];
+    }
+    __syncthreads();
+}
+
+template <int Q, int NT, int NPTS, int Ns>
+__device__ inline void fft_passes(float2* buf, const float2* __restrict__ tw, int lgTWN) {
+    if constexpr (Ns < Q) {
+        constexpr int left = Q / Ns;
+        constexpr int R = left >= 8 ? 8 : left;
+        stockham_pass<Q, NT, NPTS, Ns, R>(buf, tw, lgTWN);
+        fft_passes<Q, NT, NPTS, Ns * R>(buf, tw, lgTWN);
+    }
+}
+
+__device__ inline unsigned long long pack_best(float p, unsigned bin) {
+    if (!(p > 0.0f)) p = 0.0f;   // NaN and negatives never win
+    return (static_cast<unsigned long long>(__float_as_uint(p)) << 32) | (0xFFFFFFFFu - bin);
+}
+
+// sub-transform indices of a cell: g = 0 -> {0, P/2}, g > 0 -> {g, P - g}; -1 = none
+__device__ inline void cell_k2(int g, int P, int* ka, int* kb) {
+    if (g == 0) {
+        *ka = 0;
+        *kb = P >= 2 ? P / 2 : -1;
+    } else {
+        *ka = g;
+        *kb = P - g;
+    }
+}
+
+// ------------------------------------------------------------ column kernel
+
+struct CellInfo {
+    int valid;
+    int iL;
+    int W;
+    int dint;
+    float dfrac;
+    int cell;
+};
+
+// product of one channel at one bin from the phase-corrected value u:
+// K' |u|^4 / ((1+E) Re(u)^2 + (1-E) Im(u)^2), without the constant K'
+__device__ inline float bin_term(float ur, float ui, float ja) {
+    const float xx = 6.283185307179586f * ja * ja;
+    const float E = __expf(-xx), omE = -expm1f(-xx);
+    const float a2 = ur * ur, b2 = ui * ui;
+    const float num = a2 + b2, den = (1.0f + E) * a2 + omE * b2;
+    return den > 0.0f ? num * (num / den) : 0.0f;
+}
+
+// MULTI = false: one channel, no constraint (SMP units, MMP2 sums);
+// MULTI = true: channel sums of squares, MMP1 constraint when ud.constraint.
+template <int Q, bool MULTI>
+__global__ __launch_bounds__(Geo<Q>::NT) void k_columns(
+    const Job* __restrict__ jobs, int njobs, const ScaleDev* __restrict__ scales,
+    const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
+    const double* __restrict__ residual, unsigned long long* __restrict__ cellbest,
+    const float2* __restrict__ tw, int lgTWN, int full, unsigned long long* __restrict__ counter) {
+    using G_ = Geo<Q>;
+    constexpr int NT = G_::NT, NPTS = G_::NPTS, NCELL = G_::NCELL, EPT = G_::EPT;
+    constexpr int NACC = MULTI ? EPT : 1;
+    __shared__ float2 buf[Lds<NPTS>::SIZE];
+    __shared__ CellInfo info[NCELL];
+    __shared__ unsigned long long best[NCELL];
+    __shared__ unsigned cnt_lds;
+
+    const int tid = threadIdx.x;
+    // job lookup: last job with tile0 <= blockIdx.x
+    int lo = 0, hi = njobs - 1;
+    const int tileg = blockIdx.x;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= tileg) lo = mid; else hi = mid - 1;
+    }
+    const Job job = jobs[lo];
+    const UnitState st = state[job.unit];
+    if (!st.active) return;
+    const ScaleDev sd = scales[job.scale];
+    const UnitDev ud = units[job.unit];
+    const int cell0 = (tileg - job.tile0) * NCELL;
+    const int ncells = sd.T * sd.G;
+    const double s = sd.s, fs = sd.fs;
+
+    for (int c = tid; c < NCELL; c += NT) {
+        CellInfo ci{};
+        ci.cell = cell0 + c;
+        ci.valid = ci.cell < ncells;
+        if (ci.valid) {
+            const int t = ci.cell / sd.G;
+            const double t0 = t * sd.tMax / (sd.T - 1);
+            if (!full) ci.valid = fabs(t0 - st.tA) <= (s + st.sA) * kGaussHalfWidth;
+            const double hw = kGaussHalfWidth * s;
+            long l = lrint((t0 - hw) * fs);
+            long r = lrint((t0 + hw) * fs);
+            ci.iL = static_cast<int>(l > 0 ? l : 0);
+            const int iR = static_cast<int>(r < sd.N - 1 ? r : sd.N - 1);
+            ci.W = min(iR - ci.iL + 1, 2 * Q);
+            const double t0fixed = t0 - ci.iL / fs;
+            const double d = t0fixed * fs;
+            const double dfl = floor(d);
+            ci.dint = static_cast<int>(dfl);
+            ci.dfrac = static_cast<float>(d - dfl);
+        }
+        info[c] = ci;
+        best[c] = 0ull;
+    }
+    if (tid == 0) cnt_lds = 0;
+    int any = 0;
+    for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
+    if (!__syncthreads_or(any)) return;
+
+    const int P = sd.P, G = sd.G, M = sd.M, F = sd.F, Nfft = sd.Nfft;
+    const int shM = lgTWN - (sd.lgNfft - 1);   // tw index shift for w_M
+    const int shN = lgTWN - sd.lgNfft;         // for w_Nfft
+    const float invN = 1.0f / static_cast<float>(Nfft);
+    const float xscale = st.xscale;
+    const bool mmp1 = MULTI && ud.constraint != 0;
+    const int C = MULTI ? ud.C : 1;
+
+    float acc0[NACC], acc1[NACC], acc2[NACC];
+    float accNyq = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NACC; ++i) acc0[i] = acc1[i] = acc2[i] = 0.0f;
+    unsigned mycnt = 0;
+
+    for (int ch = 0; ch < C; ++ch) {
+        const double* x = residual + ud.res_off + static_cast<long long>(ch) * ud.N;
+        // load: packed pairs z[m] = w[2m] + i w[2m+1], pre-twiddled per slot
+#pragma unroll
+        for (int i = 0; i < EPT / 2; ++i) {
+            const int p = tid + i * NT;
+            const int c = p / Q, m = p % Q;
+            const CellInfo ci = info[c];
+            float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
+            if (ci.valid) {
+                const int n0 = 2 * m, n1 = 2 * m + 1;
+                float w0 = 0.f, w1 = 0.f;
+                if (n0 < ci.W) {
+                    const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
+                    w0 = static_cast<float>(x[ci.iL + n0] * xscale) * __expf(-sd.gA * u * u);
+                }
+                if (n1 < ci.W) {
+                    const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
+                    w1 = static_cast<float>(x[ci.iL + n1] * xscale) * __expf(-sd.gA * u * u);
+                }
+                const float2 z = make_float2(w0, w1);
+                int ka, kb;
+                cell_k2(ci.cell % G, P, &ka, &kb);
+                za = ka == 0 ? z : cmul(z, tw[(ka * m) << shM]);
+                if (kb >= 0) zb = cmul(z, tw[(kb * m) << shM]);
+            }
+            buf[lpad((2 * c) * Q + m)] = za;
+            buf[lpad((2 * c + 1) * Q + m)] = zb;
+        }
+        __syncthreads();
+        fft_passes<Q, NT, NPTS, 1>(buf, tw, lgTWN);
+
+        // post: split the packed spectrum, phase-correct, product per bin
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int e = tid + i * NT;
+            const int c = e / (2 * Q), rem = e % (2 * Q);
+            const int half = rem / Q, k1 = rem % Q;
+            const CellInfo ci = info[c];
+            if (!ci.valid) continue;
+            int ka, kb;
+            cell_k2(ci.cell % G, P, &ka, &kb);
+            const int k2 = half ? kb : ka;
+            if (k2 < 0) continue;
+            const int k = P * k1 + k2;
+            if (k >= F) continue;
+            const int slot = 2 * c + half;
+            const float2 Zk = buf[lpad(slot * Q + k1)];
+            float2 Zp;
+            if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
+            else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
+            const float2 A = make_float2(Zk.x + Zp.x, Zk.y - Zp.y);
+            const float2 B = make_float2(Zk.x - Zp.x, Zk.y + Zp.y);
+            const float2 w = tw[k << shN];
+            const float pr = w.x * B.x - w.y * B.y, pi = w.x * B.y + w.y * B.x;
+            const float Xr = 0.5f * (A.x + pi), Xi = 0.5f * (A.y - pr);
+            const bool high = 4 * k > Nfft;
+            const int j = high ? M - k : k;
+            const unsigned ci_int = (static_cast<unsigned>(j) * static_cast<unsigned>(ci.dint)) & static_cast<unsigned>(Nfft - 1);
+            float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
+            if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
+            float sn, cs;
+            sincospif(2.0f * cyc, &sn, &cs);
+            const float ur = Xr * cs - Xi * sn, ui = Xr * sn + Xi * cs;
+            if constexpr (!MULTI) {
+                ++mycnt;
+                atomicMax(&best[c], pack_best(sd.Kp * bin_term(ur, ui, static_cast<float>(j) * sd.alpha),
+                                              static_cast<unsigned>(k)));
+            } else if (mmp1 && !high) {
+                acc0[i] += ur * ur;
+                acc1[i] += ui * ui;
+                acc2[i] += ur * ui;
+            } else {
+                acc0[i] += bin_term(ur, ui, static_cast<float>(j) * sd.alpha);
+            }
+        }
+        // Nyquist bin k = M of g = 0 cells: X[M] = Re Z0 - Im Z0, u real
+        if (tid < NCELL && F > M) {
+            const CellInfo ci = info[tid];
+            if (ci.valid && ci.cell % G == 0) {
+                const float2 Z0 = buf[lpad((2 * tid) * Q)];
+                const float a = Z0.x - Z0.y;
+                accNyq += 0.5f * a * a;   // num^2/den with E = 1: a^4 / (2 a^2)
+            }
+        }
+        __syncthreads();
+    }
+
+    // final products and per-cell maximum
+    if constexpr (MULTI) {
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int e = tid + i * NT;
+            const int c = e / (2 * Q), rem = e % (2 * Q);
+            const int half = rem / Q, k1 = rem % Q;
+            const CellInfo ci = info[c];
+            if (!ci.valid) continue;
+            int ka, kb;
+            cell_k2(ci.cell % G, P, &ka, &kb);
+            const int k2 = half ? kb : ka;
+            if (k2 < 0) continue;
+            const int k = P * k1 + k2;
+            if (k >= F) continue;
+            ++mycnt;
+            const bool high = 4 * k > Nfft;
+            float p;
+            if (mmp1 && !high) {
+                const float Srr = acc0[i], Sii = acc1[i], Sri = acc2[i];
+                const float Dr = Srr - Sii, Di = 2.0f * Sri;
+                const float Dm = hypotf(Dr, Di);
+                const float lam = 0.5f * (Srr + Sii + Dm);
+                float cos2 = 1.0f;
+                if (Dm > 0.0f) cos2 = Dr >= 0.0f ? (Dm + Dr) / (2.0f * Dm) : (Di * Di) / (2.0f * Dm * (Dm - Dr));
+                const float ja = static_cast<float>(k) * sd.alpha;
+                const float xx = 6.283185307179586f * ja * ja;
+                const float E = __expf(-xx), omE = -expm1f(-xx);
+                const float den = omE + 2.0f * E * cos2;
+                p = den > 0.0f ? sd.Kp * lam / den : 0.0f;
+            } else {
+                p = sd.Kp * acc0[i];
+            }
+            atomicMax(&best[c], pack_best(p, static_cast<unsigned>(k)));
+        }
+    }
+    if (tid < NCELL && F > M) {
+        const CellInfo ci = info[tid];
+        if (ci.valid && ci.cell % G == 0) {
+            ++mycnt;
+            atomicMax(&best[tid], pack_best(sd.Kp * accNyq, static_cast<unsigned>(M)));
+        }
+    }
+    if (mycnt) atomicAdd(&cnt_lds, mycnt);
+    __syncthreads();
+    for (int c = tid; c < NCELL; c += NT) {
+        if (info[c].valid) cellbest[job.cell_off + info[c].cell] = best[c];
+    }
+    if (tid == 0 && counter) {
+        constexpr int cls = ilog2c(Q) - 1;
+        atomicAdd(&counter[2 * cls], static_cast<unsigned long long>(cnt_lds));
+        atomicAdd(&counter[2 * cls + 1], static_cast<unsigned long long>(cnt_lds) * ud.C);
+    }
+}
+
+// ------------------------------------------------------------ unit kernels
+
+template <int NT>
+__device__ double block_sum(double v, double* red) {
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// residual energy sum_c (sum_n x^2) / fs (base.hpp:39-58)
+template <int NT>
+__device__ double unit_energy(const double* x, int C, int N, double fs, double* red) {
+    double total = 0.0;
+    for (int c = 0; c < C; ++c) {
+        double part = 0.0;
+        for (int n = threadIdx.x; n < N; n += NT) {
+            const double v = x[static_cast<long long>(c) * N + n];
+            part += v * v;
+        }
+        total += block_sum<NT>(part, red) / fs;
+    }
+    return total;
+}
+
+// residual <- input (FP64), energy, input scaling; one workgroup per unit
+__global__ __launch_bounds__(kSelectThreads) void k_init(const UnitDev* __restrict__ units,
+                                                          UnitState* __restrict__ state,
+                                                          const float* __restrict__ input,
+                                                          double* __restrict__ residual) {
+    __shared__ double red[kSelectThreads];
+    const UnitDev ud = units[blockIdx.x];
+    double* x = residual + ud.res_off;
+    for (int c = 0; c < ud.C; ++c) {
+        for (int n = threadIdx.x; n < ud.N; n += kSelectThreads) {
+            double v;
+            if (ud.in_sum) {   // mmp.cpp:24-32: channel 0, then += channels 1..C-1
+                v = input[ud.in_off + n];
+                for (int k = 1; k < ud.in_sum; ++k) v += static_cast<double>(input[ud.in_off + static_cast<long long>(k) * ud.N + n]);
+            } else {
+                v = input[ud.in_off + static_cast<long long>(ud.in_channel + c) * ud.N + n];
+            }
+            x[static_cast<long long>(c) * ud.N + n] = v;
+        }
+    }
+    __syncthreads();
+    const double E = unit_energy<kSelectThreads>(x, ud.C, ud.N, ud.fs, red);
+    if (threadIdx.x == 0) {
+        UnitState s{};
+        s.totalEnergy = E;
+        s.residEnergy = E;
+        s.iteration = 0;
+        s.active = E != 0.0;
+        s.status = E != 0.0 ? EMPI_OK : EMPI_SIGNAL_IS_EMPTY;
+        // FP32 maps work on x * 2^e with the RMS near 1 (exact scaling)
+        const double ms = E * ud.fs / (static_cast<double>(ud.C) * ud.N);
+        int ex = 0;
+        if (ms > 0) std::frexp(sqrt(ms), &ex);
+        ex = max(-100, min(100, ex));
+        s.xscale = ldexpf(1.0f, -ex);
+        state[blockIdx.x] = s;
+        ud.energies[0] = E;
+    }
+}
+
+// FP64 value of map entry (k, f, t) of every channel (gabor.cpp:279-303 at one bin)
+template <int NT>
+__device__ void fit_column(const UnitDev& ud, const ScaleDev& sd, const double* x, int f, int t,
+                           double* red, double2* out) {
+    const double fs = sd.fs, s = sd.s;
+    const double t0 = t * sd.tMax / (sd.T - 1);
+    const double hw = kGaussHalfWidth * s;
+    long l = lrint((t0 - hw) * fs), r = lrint((t0 + hw) * fs);
+    const int iL = static_cast<int>(l > 0 ? l : 0);
+    const int iR = static_cast<int>(r < sd.N - 1 ? r : sd.N - 1);
+    const int W = min(iR - iL + 1, sd.Nfft);
+    const double t0fixed = t0 - iL / fs;
+    const double gw = sqrt(M_PI) / s;
+    const double ga = gw * (1.0 / fs), gb = gw * t0fixed;
+    const double norm = 1.0 / fs;
+    const double ang = (f * fs / sd.Nfft) * (2 * M_PI * t0fixed);
+    const double cr = cos(ang), si = sin(ang);
+    const unsigned mask = static_cast<unsigned>(sd.Nfft - 1);
+    for (int c0 = 0; c0 < ud.C; c0 += 4) {
+        double pr[4] = {0, 0, 0, 0}, pi[4] = {0, 0, 0, 0};
+        for (int n = threadIdx.x; n < W; n += NT) {
+            const double xw = ga * n - gb;
+            const double g = exp(-xw * xw);
+            const unsigned idx = (static_cast<unsigned>(f) * static_cast<unsigned>(n)) & mask;
+            double sn, cs;
+            sincospi(-2.0 * static_cast<double>(idx) / sd.Nfft, &sn, &cs);
+            const double gr = g * cs, gi = g * sn;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (c0 + q < ud.C) {
+                    const double xv = x[static_cast<long long>(c0 + q) * sd.N + iL + n];
+                    pr[q] += xv * gr;
+                    pi[q] += xv * gi;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double Xr = block_sum<NT>(pr[q], red);
+            const double Xi = block_sum<NT>(pi[q], red);
+            if (threadIdx.x == 0 && c0 + q < ud.C) {
+                const double vr = norm * Xr, vi = norm * Xi;
+                out[c0 + q] = make_double2(vr * cr - vi * si, vr * si + vi * cr);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Atom parameters from the (constrained) values (gabor.cpp:111-131, mmp.cpp:9-21).
+__device__ void atom_params(const UnitDev& ud, const ScaleDev& sd, int k, int f, int t, double2* v,
+                            empi_atom* out, int it) {
+    const double fs = sd.fs, s = sd.s;
+    const double normComplex = sqrt(M_SQRT2 / s);
+    const double freqNyquist = 0.5 * fs;
+    const double fv = f * fs / sd.Nfft;
+    const bool isHigh = fv > 0.5 * freqNyquist;
+    const double f4 = isHigh ? (freqNyquist - fv) : fv;
+    const double expFactor = exp(-2 * M_PI * s * s * f4 * f4);
+    const double tt = t * sd.tMax / (sd.T - 1);
+    if (ud.constraint && !isHigh) {
+        double dr = 0.0, di = 0.0;
+        for (int c = 0; c < ud.C; ++c) {
+            const double re = v[c].x, im = v[c].y;
+            dr += re * re - im * im;
+            di += re * im + im * re;
+        }
+        const double angle = 0.5 * atan2(di, dr);
+        const double ca = cos(angle), sa = sin(angle);
+        for (int c = 0; c < ud.C; ++c) {
+            const double re = v[c].x, im = v[c].y;
+            const double rr = hypot(re, im) * cos(atan2(im, re) - angle);
+            v[c] = make_double2(rr * ca, rr * sa);
+        }
+    }
+    for (int c = 0; c < ud.C; ++c) {
+        const double re = v[c].x, im = v[c].y;
+        const double phase = atan2(im, re);
+        const double phase4 = isHigh ? (2 * M_PI * freqNyquist * tt - phase) : phase;
+        const double normReal = M_SQRT2 * normComplex / sqrt(1.0 + cos(2 * phase4) * expFactor);
+        const double mag = hypot(re, im);
+        empi_atom a;
+        a.params[0] = mag * normReal * sqrt(fs);
+        a.params[1] = mag * normReal * normReal;
+        a.params[2] = tt * fs;
+        a.params[3] = s * fs;
+        a.params[4] = fv * 2.0 / fs;
+        a.params[5] = phase;
+        a.scale = k;
+        a.freq = f;
+        a.time = t;
+        a.reserved = 0;
+        out[static_cast<long long>(c) * ud.atom_cstride + it] = a;
+    }
+}
+
+// x -= atom waveform (gabor.cpp:143-187 with fit = false); all threads
+template <int NT>
+__device__ void subtract_waveform(const empi_atom& a, double fs, double* x, int N) {
+    const double sA = a.params[3] / fs;
+    const double fA = a.params[4] * fs / 2.0;
+    const double tA = a.params[2] / fs;
+    const double amplitude = a.params[1];
+    const double omega = 2.0 * M_PI * fA;
+    const double phase = a.params[5];
+    const double hw = kGaussHalfWidth * sA;
+    long l = lrint((tA - hw) * fs), r = lrint((tA + hw) * fs);
+    const int iL = static_cast<int>(l > 0 ? l : 0);
+    const int iR = static_cast<int>(r < N - 1 ? r : N - 1);
+    const int Nw = 1 + iR - iL;
+    const double gw = sqrt(M_PI) / sA;
+    const double ga = gw * (1.0 / fs), gb = gw * (tA - iL / fs);
+    for (int i = threadIdx.x; i < Nw; i += NT) {
+        const double t = (i + iL) / fs - tA;
+        double w = amplitude * cos(omega * t + phase);
+        const double xw = ga * i - gb;
+        w *= exp(-xw * xw);
+        x[iL + i] -= w;
+    }
+}
+
+struct SelBest {
+    float p;
+    unsigned long long key;   // (scale << 52) | (bin << 26) | t
+};
+__device__ inline bool sel_better(const SelBest& a, const SelBest& b) {
+    return a.p > b.p || (a.p == b.p && a.key < b.key);
+}
+
+// mode bit 0: find the best atom; bit 1: subtract it and test the stop rule
+template <int NT>
+__global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units, UnitState* __restrict__ state,
+                                               const ScaleDev* __restrict__ scales,
+                                               const unsigned long long* __restrict__ cellbest,
+                                               double* __restrict__ residual, int mode) {
+    __shared__ double red[NT];
+    __shared__ float redp[NT];
+    __shared__ unsigned long long redk[NT];
+    __shared__ empi_atom chosen;
+    const int u = blockIdx.x;
+    const int tid = threadIdx.x;
+    UnitState st = state[u];
+    if (!st.active) return;
+    const UnitDev ud = units[u];
+    const int it = st.iteration;
+
+    // 1. argmax over the cell bests: product desc, then scale, bin, time asc
+    SelBest b{-1.0f, ~0ull};
+    long long base = ud.cell_off;
+    for (int l = 0; l < ud.L; ++l) {
+        const ScaleDev& sd = scales[ud.scale0 + l];
+        const int n = sd.T * sd.G;
+        for (int i = tid; i < n; i += NT) {
+            const unsigned long long v = cellbest[base + i];
+            SelBest c;
+            c.p = __uint_as_float(static_cast<unsigned>(v >> 32));
+            const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(v & 0xFFFFFFFFu);
+            c.key = (static_cast<unsigned long long>(l) << 52) | (static_cast<unsigned long long>(bin) << 26) |
+                    static_cast<unsigned long long>(i / sd.G);
+            if (sel_better(c, b)) b = c;
+        }
+        base += n;
+    }
+    redp[tid] = b.p;
+    redk[tid] = b.key;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            SelBest o{redp[tid + s], redk[tid + s]};
+            SelBest m{redp[tid], redk[tid]};
+            if (sel_better(o, m)) {
+                redp[tid] = o.p;
+                redk[tid] = o.key;
+            }
+        }
+        __syncthreads();
+    }
+    const float bp = redp[0];
+    const unsigned long long bk = redk[0];
+    __syncthreads();
+    if (!(bp > 0.0f)) {
+        if (tid == 0) {
+            st.status = EMPI_ENERGY_MAP_IS_EMPTY;
+            st.active = 0;
+            state[u] = st;
+        }
+        return;
+    }
+    const int k = static_cast<int>(bk >> 52);
+    const int f = static_cast<int>((bk >> 26) & ((1ull << 26) - 1));
+    const int t = static_cast<int>(bk & ((1ull << 26) - 1));
+    const ScaleDev sd = scales[ud.scale0 + k];
+    double* x = residual + ud.res_off;
+
+    // 2. FP64 refit of the chosen column, 3. parameters
+    fit_column<NT>(ud, sd, x, f, t, red, ud.fitv);
+    if (tid == 0) atom_params(ud, sd, k, f, t, ud.fitv, ud.atoms, it);
+    __syncthreads();
+    if (!(mode & 2)) {
+        if (tid == 0) {
+            st.iteration = it + 1;
+            state[u] = st;
+        }
+        return;
+    }
+    // classes.cpp:57-59: the last atom is not subtracted
+    if (it + 1 >= ud.iterMax) {
+        if (tid == 0) {
+            st.iteration = it + 1;
+            st.active = 0;
+            state[u] = st;
+        }
+        return;
+    }
+    // 4. residual update per channel, 5. energy and stop test (classes.cpp:61-70)
+    for (int c = 0; c < ud.C; ++c) {
+        if (tid == 0) chosen = ud.atoms[static_cast<long long>(c) * ud.atom_cstride + it];
+        __syncthreads();
+        subtract_waveform<NT>(chosen, ud.fs, x + static_cast<long long>(c) * ud.N, ud.N);
+        __syncthreads();
+    }
+    const double E = unit_energy<NT>(x, ud.C, ud.N, ud.fs, red);
+    if (tid == 0) {
+        const empi_atom& a = ud.atoms[it];
+        st.iteration = it + 1;
+        st.residEnergy = E;
+        st.tA = a.params[2] / ud.fs;
+        st.sA = a.params[3] / ud.fs;
+        ud.energies[it + 1] = E;
+        if (E / st.totalEnergy <= ud.residualEnergy) st.active = 0;
+        state[u] = st;
+    }
+}
+
+// MMP2 per-channel least-squares refit of the sum's atoms (mmp.cpp:36-44)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_refit(const empi_atom* __restrict__ sum_atoms,
+                                              const UnitState* __restrict__ sum_state,
+                                              const float* __restrict__ input, long long in_off, int C,
+                                              int N, double fs, double* __restrict__ scratch,
+                                              empi_atom* __restrict__ out, int cstride) {
+    __shared__ double red[NT];
+    __shared__ empi_atom a;
+    const int c = blockIdx.x;
+    double* x = scratch + static_cast<long long>(c) * N;
+    for (int n = threadIdx.x; n < N; n += NT) x[n] = input[in_off + static_cast<long long>(c) * N + n];
+    __syncthreads();
+    const int count = sum_state->iteration;
+    for (int i = 0; i < count; ++i) {
+        if (threadIdx.x == 0) a = sum_atoms[i];
+        __syncthreads();
+        const double sA = a.params[3] / fs, fA = a.params[4] * fs / 2.0, tA = a.params[2] / fs;
+        const double omega = 2.0 * M_PI * fA, phase = a.params[5];
+        const double hw = kGaussHalfWidth * sA;
+        long l = lrint((tA - hw) * fs), r = lrint((tA + hw) * fs);
+        const int iL = static_cast<int>(l > 0 ? l : 0);
+        const int iR = static_cast<int>(r < N - 1 ? r : N - 1);
+        const int Nw = 1 + iR - iL;
+        const double gw = sqrt(M_PI) / sA;
+        const double ga = gw * (1.0 / fs), gb = gw * (tA - iL / fs);
+        double n2 = 0.0;
+        for (int j = threadIdx.x; j < Nw; j += NT) {
+            const double t = (j + iL) / fs - tA;
+            const double xw = ga * j - gb;
+            const double w = cos(omega * t + phase) * exp(-xw * xw);
+            n2 += w * w;
+        }
+        const double norm = sqrt(block_sum<NT>(n2, red));
+        double dot = 0.0;
+        for (int j = threadIdx.x; j < Nw; j += NT) {
+            const double t = (j + iL) / fs - tA;
+            const double xw = ga * j - gb;
+            const double w = cos(omega * t + phase) * exp(-xw * xw) / norm;
+            dot += w * x[iL + j];
+        }
+        const double modulus = block_sum<NT>(dot, red);
+        for (int j = threadIdx.x; j < Nw; j += NT) {
+            const double t = (j + iL) / fs - tA;
+            const double xw = ga * j - gb;
+            const double w = cos(omega * t + phase) * exp(-xw * xw) / norm;
+            x[iL + j] -= w * modulus;
+        }
+        if (threadIdx.x == 0) {
+            empi_atom o = a;
+            o.params[0] = modulus;
+            o.params[1] = modulus / norm;
+            out[static_cast<long long>(c) * cstride + i] = o;
+        }
+        __syncthreads();
+    }
+}
+
+// subtract one given atom from channel c of a unit (per-call workspace API)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_subtract_one(const UnitDev* __restrict__ units, UnitState* __restrict__ state,
+                                                     const empi_atom* __restrict__ atom, int c,
+                                                     double* __restrict__ residual) {
+    const UnitDev ud = units[0];
+    __shared__ empi_atom a;
+    if (threadIdx.x == 0) a = *atom;
+    __syncthreads();
+    subtract_waveform<NT>(a, ud.fs, residual + ud.res_off + static_cast<long long>(c) * ud.N, ud.N);
+    if (threadIdx.x == 0) {
+        state[0].tA = a.params[2] / ud.fs;
+        state[0].sA = a.params[3] / ud.fs;
+    }
+}
+
+// ------------------------------------------------------------ host plan
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    empi_status alloc(size_t count) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = count;
+        if (!count) return EMPI_OK;
+        hipError_t e = hipMalloc(&p, sizeof(T) * count);
+        if (e != hipSuccess) {
+            p = nullptr;
+            n = 0;
+            return e == hipErrorOutOfMemory ? EMPI_NO_MEMORY : EMPI_DEVICE;
+        }
+        return EMPI_OK;
+    }
+    empi_status upload(const T* h, size_t count, hipStream_t s) {
+        if (!count) return EMPI_OK;
+        EMPI_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+        return EMPI_OK;
+    }
+};
+
+struct ClassJobs {
+    std::vector<Job> jobs;
+    int tiles = 0;
+    DevBuf<Job> d;
+};
+
+struct ItemInfo {
+    empi_item item;
+    int dict;
+    long long in_off;
+    std::vector<int> units;   // unit indices
+    int rows;                 // energy rows
+    long long atom_off;       // atoms offset: [C][I]
+    long long energy_off;     // energies offset: [rows][I+1]
+    long long sum_atom_off;   // MMP2: atoms of the channel sum [I]
+    long long scratch_off;    // MMP2: refit scratch [C][N]
+};
+
+constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
+
+typedef void (*ColumnLauncher)(int tiles, hipStream_t s, const Job* jobs, int njobs, const ScaleDev* scales,
+                               const UnitDev* units, const UnitState* state, const double* residual,
+                               unsigned long long* cellbest, const float2* tw, int lgTWN, int full,
+                               unsigned long long* counter);
+
+template <int Q, bool MULTI>
+void launch_columns(int tiles, hipStream_t s, const Job* jobs, int njobs, const ScaleDev* scales,
+                    const UnitDev* units, const UnitState* state, const double* residual,
+                    unsigned long long* cellbest, const float2* tw, int lgTWN, int full,
+                    unsigned long long* counter) {
+    hipLaunchKernelGGL((k_columns<Q, MULTI>), dim3(tiles), dim3(Geo<Q>::NT), 0, s, jobs, njobs, scales, units,
+                       state, residual, cellbest, tw, lgTWN, full, counter);
+}
+
+// [multi][class]
+const ColumnLauncher kLaunchers[2][kNumClasses] = {
+    {launch_columns<2, false>, launch_columns<4, false>, launch_columns<8, false>, launch_columns<16, false>,
+     launch_columns<32, false>, launch_columns<64, false>, launch_columns<128, false>, launch_columns<256, false>,
+     launch_columns<512, false>, launch_columns<1024, false>, launch_columns<2048, false>,
+     launch_columns<4096, false>, launch_columns<8192, false>},
+    {launch_columns<2, true>, launch_columns<4, true>, launch_columns<8, true>, launch_columns<16, true>,
+     launch_columns<32, true>, launch_columns<64, true>, launch_columns<128, true>, launch_columns<256, true>,
+     launch_columns<512, true>, launch_columns<1024, true>, launch_columns<2048, true>, launch_columns<4096, true>,
+     launch_columns<8192, true>}};
+
+int cells_per_wg(int Q) {
+    const int nsl = Q >= 2048 ? 2 : 4096 / Q;
+    return nsl / 2;
+}
+
+}  // namespace
+
+struct empi_plan {
+    std::vector<ItemInfo> items;
+    std::vector<Dictionary> dicts;
+    std::vector<int> dict_scale0;
+    std::vector<UnitDev> units;
+    std::vector<ScaleDev> scales_h;
+    ClassJobs classes[2][kNumClasses];   // [multi][class]
+    int maxIter = 0;
+    int twN = 2, lgTWN = 1;
+    long long total_in = 0;
+
+    DevBuf<ScaleDev> d_scales;
+    DevBuf<UnitDev> d_units;
+    DevBuf<UnitState> d_state;
+    DevBuf<double> d_res;
+    DevBuf<unsigned long long> d_cells;
+    DevBuf<float2> d_tw;
+    DevBuf<empi_atom> d_atoms;
+    DevBuf<double> d_energies;
+    DevBuf<double2> d_fitv;
+    DevBuf<double> d_scratch;
+    DevBuf<float> d_input;
+    DevBuf<unsigned long long> d_counter;
+
+    // host copies of the last run's results
+    std::vector<empi_atom> h_atoms;
+    std::vector<double> h_energies;
+    std::vector<UnitState> h_state;
+    unsigned long long h_counter[2 * kNumClasses] = {};
+    bool has_results = false;
+
+    // optional per-class kernel timing (HIP events on the launch stream)
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[kNumClasses];
+    size_t ev_used[kNumClasses] = {};
+    double class_ms[kNumClasses] = {};
+    long long class_launches[kNumClasses] = {};
+    ~empi_plan() {
+        for (auto& v : ev)
+            for (auto& e : v) {
+                (void)hipEventDestroy(e.first);
+                (void)hipEventDestroy(e.second);
+            }
+    }
+};
+
+struct empi_ws {
+    empi_plan plan;
+    int C = 0, N = 0;
+    double fs = 0;
+    bool computed = false;
+    bool pending = false;    // columns to re-evaluate before the next search
+    DevBuf<empi_atom> d_one;
+};
+
+namespace {
+
+int lg2(long long x) {
+    int l = 0;
+    while ((1ll << l) < x) ++l;
+    return l;
+}
+
+empi_status dict_for(empi_plan* p, const empi_dict_params& dp, double fs, int N, int* id) {
+    for (size_t i = 0; i < p->dicts.size(); ++i) {
+        const Dictionary& d = p->dicts[i];
+        if (d.fs == fs && d.N == N && d.params.energyError == dp.energyError && d.params.scaleMin == dp.scaleMin &&
+            d.params.scaleMax == dp.scaleMax &&
+            (d.params.freqMax == dp.freqMax || (std::isnan(d.params.freqMax) && std::isnan(dp.freqMax)))) {
+            *id = static_cast<int>(i);
+            return EMPI_OK;
+        }
+    }
+    Dictionary d;
+    empi_status st = build_dictionary(dp, fs, N, &d);
+    if (st != EMPI_OK) return st;
+    for (const Scale& sc : d.scales) {
+        if (sc.Q > kMaxQ) return EMPI_UNSUPPORTED;
+    }
+    *id = static_cast<int>(p->dicts.size());
+    p->dicts.push_back(std::move(d));
+    return EMPI_OK;
+}
+
+// Lays out units, scales, jobs and buffers; items may already carry per-unit
+// FP64 input (workspace mode) in which case in_sum/in_channel are unused.
+empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_constraint) {
+    if (count <= 0 || !items) return EMPI_INVALID_ARGUMENT;
+    long long in_off = 0, res_off = 0, cell_off = 0, atom_off = 0, energy_off = 0, scratch_off = 0;
+    int max_c = 1;
+    for (int i = 0; i < count; ++i) {
+        const empi_item& it = items[i];
+        if (it.channels <= 0 || it.samples < 2 || !(it.fs > 0) || it.iterationMax <= 0 || it.variant < EMPI_SMP ||
+            it.variant > EMPI_MMP3)
+            return EMPI_INVALID_ARGUMENT;
+        const double residualEnergy = 1 - 0.01 * it.energyPercent;
+        if (residualEnergy < 0 || residualEnergy >= 1.0) return EMPI_INVALID_ARGUMENT;
+        int did = 0;
+        empi_status st = dict_for(p, it.dict, it.fs, it.samples, &did);
+        if (st != EMPI_OK) return st;
+        ItemInfo info{};
+        info.item = it;
+        info.dict = did;
+        info.in_off = in_off;
+        const int C = it.channels, N = it.samples, I = it.iterationMax;
+        info.rows = it.variant == EMPI_SMP ? C : 1;
+        info.atom_off = atom_off;
+        atom_off += static_cast<long long>(C) * I;
+        info.energy_off = energy_off;
+        energy_off += static_cast<long long>(info.rows) * (I + 1);
+        info.sum_atom_off = -1;
+        info.scratch_off = -1;
+        if (it.variant == EMPI_MMP2) {
+            info.sum_atom_off = atom_off;
+            atom_off += I;
+            info.scratch_off = scratch_off;
+            scratch_off += static_cast<long long>(C) * N;
+        }
+        const Dictionary& d = p->dicts[did];
+        auto add_unit = [&](int uc, int in_channel, int in_sum, long long aoff, int cstride, long long eoff,
+                            int constraint) {
+            UnitDev u{};
+            u.res_off = res_off;
+            res_off += static_cast<long long>(uc) * N;
+            u.cell_off = cell_off;
+            cell_off += d.cells;
+            u.in_off = in_off;
+            u.C = uc;
+            u.N = N;
+            u.in_channel = in_channel;
+            u.in_sum = in_sum;
+            u.scale0 = did;   // patched to a scale index below
+            u.L = static_cast<int>(d.scales.size());
+            u.constraint = constraint;
+            u.iterMax = I;
+            u.fs = it.fs;
+            u.residualEnergy = residualEnergy;
+            u.atoms = reinterpret_cast<empi_atom*>(aoff);      // patched to a device pointer below
+            u.atom_cstride = cstride;
+            u.energies = reinterpret_cast<double*>(eoff);      // patched below
+            info.units.push_back(static_cast<int>(p->units.size()));
+            p->units.push_back(u);
+            max_c = std::max(max_c, uc);
+        };
+        if (it.variant == EMPI_SMP || (C == 1 && it.variant != EMPI_MMP2)) {
+            if (it.variant == EMPI_SMP) {
+                for (int c = 0; c < C; ++c)
+                    add_unit(1, c, 0, info.atom_off + static_cast<long long>(c) * I, I,
+                             info.energy_off + static_cast<long long>(c) * (I + 1), 0);
+            } else {
+                add_unit(1, 0, 0, info.atom_off, I, info.energy_off,
+                         it.variant == EMPI_MMP1 ? 1 : ws_constraint);
+            }
+        } else if (it.variant == EMPI_MMP2) {
+            add_unit(1, 0, C, info.sum_atom_off, I, info.energy_off, 0);
+        } else {
+            add_unit(C, 0, 0, info.atom_off, I, info.energy_off, it.variant == EMPI_MMP1 ? 1 : 0);
+        }
+        in_off += static_cast<long long>(C) * N;
+        p->maxIter = std::max(p->maxIter, I);
+        p->items.push_back(std::move(info));
+    }
+    p->total_in = in_off;
+
+    // scales of every dictionary
+    int maxNfft = 2;
+    for (size_t di = 0; di < p->dicts.size(); ++di) {
+        const Dictionary& d = p->dicts[di];
+        p->dict_scale0.push_back(static_cast<int>(p->scales_h.size()));
+        long long before = 0;
+        for (const Scale& sc : d.scales) {
+            ScaleDev s{};
+            s.s = sc.s;
+            s.fs = d.fs;
+            s.tMax = d.tMax;
+            s.cells_before = before;
+            before += static_cast<long long>(sc.T) * sc.G;
+            s.N = d.N;
+            s.Nfft = sc.Nfft;
+            s.M = sc.Nfft / 2;
+            s.T = sc.T;
+            s.F = sc.F;
+            s.Q = sc.Q;
+            s.P = sc.P;
+            s.G = sc.G;
+            s.lgQ = lg2(sc.Q);
+            s.lgNfft = lg2(sc.Nfft);
+            s.alpha = static_cast<float>(sc.s * d.fs / sc.Nfft);
+            s.Kp = static_cast<float>(2.0 * M_SQRT2 / (sc.s * d.fs));
+            s.gA = static_cast<float>(M_PI / ((sc.s * d.fs) * (sc.s * d.fs)));
+            s.Wmax = sc.Wmax;
+            maxNfft = std::max(maxNfft, sc.Nfft);
+            p->scales_h.push_back(s);
+        }
+    }
+    p->twN = maxNfft;
+    p->lgTWN = lg2(maxNfft);
+
+    // jobs per class
+    for (size_t ui = 0; ui < p->units.size(); ++ui) {
+        UnitDev& u = p->units[ui];
+        const int did = u.scale0;
+        u.scale0 = p->dict_scale0[did];
+        for (int l = 0; l < u.L; ++l) {
+            const ScaleDev& s = p->scales_h[u.scale0 + l];
+            const int cls = s.lgQ - kMinLgQ;
+            const int multi = (u.C > 1 || u.constraint) ? 1 : 0;
+            ClassJobs& cj = p->classes[multi][cls];
+            Job j{};
+            j.unit = static_cast<int>(ui);
+            j.scale = u.scale0 + l;
+            j.cell_off = u.cell_off + s.cells_before;
+            const int cpw = cells_per_wg(s.Q);
+            j.tiles = static_cast<int>((static_cast<long long>(s.T) * s.G + cpw - 1) / cpw);
+            j.tile0 = cj.tiles;
+            if (static_cast<long long>(cj.tiles) + j.tiles > INT_MAX) return EMPI_UNSUPPORTED;
+            cj.tiles += j.tiles;
+            cj.jobs.push_back(j);
+        }
+    }
+
+    // device buffers
+    empi_status st;
+    if ((st = p->d_scales.alloc(p->scales_h.size())) != EMPI_OK) return st;
+    if ((st = p->d_units.alloc(p->units.size())) != EMPI_OK) return st;
+    if ((st = p->d_state.alloc(p->units.size())) != EMPI_OK) return st;
+    if ((st = p->d_res.alloc(static_cast<size_t>(res_off))) != EMPI_OK) return st;
+    if ((st = p->d_cells.alloc(static_cast<size_t>(cell_off))) != EMPI_OK) return st;
+    if ((st = p->d_tw.alloc(static_cast<size_t>(p->twN))) != EMPI_OK) return st;
+    if ((st = p->d_atoms.alloc(static_cast<size_t>(atom_off))) != EMPI_OK) return st;
+    if ((st = p->d_energies.alloc(static_cast<size_t>(energy_off))) != EMPI_OK) return st;
+    if ((st = p->d_fitv.alloc(p->units.size() * static_cast<size_t>(max_c))) != EMPI_OK) return st;
+    if ((st = p->d_scratch.alloc(static_cast<size_t>(scratch_off))) != EMPI_OK) return st;
+    if ((st = p->d_counter.alloc(2 * kNumClasses)) != EMPI_OK) return st;
+    for (size_t ui = 0; ui < p->units.size(); ++ui) {
+        UnitDev& u = p->units[ui];
+        u.atoms = p->d_atoms.p + reinterpret_cast<long long>(u.atoms);
+        u.energies = p->d_energies.p + reinterpret_cast<long long>(u.energies);
+        u.fitv = p->d_fitv.p + ui * static_cast<size_t>(max_c);
+    }
+    std::vector<float2> tw(static_cast<size_t>(p->twN));
+    for (int j = 0; j < p->twN; ++j) {
+        const double a = -2.0 * M_PI * static_cast<double>(j) / static_cast<double>(p->twN);
+        tw[j] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+    }
+    hipStream_t s0 = nullptr;
+    if ((st = p->d_scales.upload(p->scales_h.data(), p->scales_h.size(), s0)) != EMPI_OK) return st;
+    if ((st = p->d_units.upload(p->units.data(), p->units.size(), s0)) != EMPI_OK) return st;
+    if ((st = p->d_tw.upload(tw.data(), tw.size(), s0)) != EMPI_OK) return st;
+    for (auto& row : p->classes) {
+        for (ClassJobs& cj : row) {
+            if (cj.jobs.empty()) continue;
+            if ((st = cj.d.alloc(cj.jobs.size())) != EMPI_OK) return st;
+            if ((st = cj.d.upload(cj.jobs.data(), cj.jobs.size(), s0)) != EMPI_OK) return st;
+        }
+    }
+    EMPI_HIP(hipStreamSynchronize(s0));
+    return EMPI_OK;
+}
+
+empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
+    for (int mc = 0; mc < 2 * kNumClasses; ++mc) {
+        const int multi = mc / kNumClasses, c = mc % kNumClasses;
+        ClassJobs& cj = p->classes[multi][c];
+        if (cj.jobs.empty() || cj.tiles == 0) continue;
+        std::pair<hipEvent_t, hipEvent_t>* evp = nullptr;
+        if (p->profiling) {
+            if (p->ev_used[c] == p->ev[c].size()) {
+                std::pair<hipEvent_t, hipEvent_t> e;
+                EMPI_HIP(hipEventCreate(&e.first));
+                EMPI_HIP(hipEventCreate(&e.second));
+                p->ev[c].push_back(e);
+            }
+            evp = &p->ev[c][p->ev_used[c]++];
+            EMPI_HIP(hipEventRecord(evp->first, s));
+        }
+        kLaunchers[multi][c](cj.tiles, s, cj.d.p, static_cast<int>(cj.jobs.size()), p->d_scales.p, p->d_units.p,
+                      p->d_state.p, p->d_res.p, p->d_cells.p, p->d_tw.p, p->lgTWN, full, p->d_counter.p);
+        if (evp) EMPI_HIP(hipEventRecord(evp->second, s));
+    }
+    EMPI_HIP(hipGetLastError());
+    return EMPI_OK;
+}
+
+empi_status collect_profile(empi_plan* p) {
+    for (int c = 0; c < kNumClasses; ++c) {
+        p->class_ms[c] = 0.0;
+        p->class_launches[c] = 0;
+        for (size_t i = 0; i < p->ev_used[c]; ++i) {
+            float ms = 0.0f;
+            EMPI_HIP(hipEventElapsedTime(&ms, p->ev[c][i].first, p->ev[c][i].second));
+            p->class_ms[c] += ms;
+            p->class_launches[c] += 1;
+        }
+        p->ev_used[c] = 0;
+    }
+    return EMPI_OK;
+}
+
+empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream_t s) {
+    const float* in = signal;
+    empi_status st;
+    if (!on_device) {
+        if (!p->d_input.p || p->d_input.n < static_cast<size_t>(p->total_in)) {
+            if ((st = p->d_input.alloc(static_cast<size_t>(p->total_in))) != EMPI_OK) return st;
+        }
+        if ((st = p->d_input.upload(signal, static_cast<size_t>(p->total_in), s)) != EMPI_OK) return st;
+        in = p->d_input.p;
+    }
+    const int U = static_cast<int>(p->units.size());
+    EMPI_HIP(hipMemsetAsync(p->d_counter.p, 0, sizeof(unsigned long long) * 2 * kNumClasses, s));
+    EMPI_HIP(hipMemsetAsync(p->d_atoms.p, 0, sizeof(empi_atom) * p->d_atoms.n, s));
+    for (int c = 0; c < kNumClasses; ++c) p->ev_used[c] = 0;
+    {
+        std::vector<double> nanv(p->d_energies.n, NAN);
+        EMPI_HIP(hipMemcpyAsync(p->d_energies.p, nanv.data(), sizeof(double) * nanv.size(), hipMemcpyHostToDevice, s));
+        EMPI_HIP(hipStreamSynchronize(s));
+    }
+    hipLaunchKernelGGL(k_init, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p, in, p->d_res.p);
+    EMPI_HIP(hipGetLastError());
+    if ((st = launch_all_columns(p, s, 1)) != EMPI_OK) return st;
+    std::vector<UnitState> hs(U);
+    for (int it = 0; it < p->maxIter; ++it) {
+        hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
+                           p->d_state.p, p->d_scales.p, p->d_cells.p, p->d_res.p, 3);
+        EMPI_HIP(hipGetLastError());
+        if (it + 1 < p->maxIter) {
+            if ((st = launch_all_columns(p, s, 0)) != EMPI_OK) return st;
+        }
+        if ((it & 7) == 7) {
+            EMPI_HIP(hipMemcpyAsync(hs.data(), p->d_state.p, sizeof(UnitState) * U, hipMemcpyDeviceToHost, s));
+            EMPI_HIP(hipStreamSynchronize(s));
+            bool any = false;
+            for (const UnitState& x : hs) any |= x.active != 0;
+            if (!any) break;
+        }
+    }
+    // MMP2 refits
+    for (const ItemInfo& info : p->items) {
+        if (info.sum_atom_off < 0) continue;
+        const UnitDev& u = p->units[info.units[0]];
+        (void)u;
+        hipLaunchKernelGGL(k_refit<kSelectThreads>, dim3(info.item.channels), dim3(kSelectThreads), 0, s,
+                           p->d_atoms.p + info.sum_atom_off, p->d_state.p + info.units[0], in, info.in_off,
+                           info.item.channels, info.item.samples, info.item.fs, p->d_scratch.p + info.scratch_off,
+                           p->d_atoms.p + info.atom_off, info.item.iterationMax);
+        EMPI_HIP(hipGetLastError());
+    }
+    p->h_atoms.resize(p->d_atoms.n);
+    p->h_energies.resize(p->d_energies.n);
+    p->h_state.resize(U);
+    EMPI_HIP(hipMemcpyAsync(p->h_atoms.data(), p->d_atoms.p, sizeof(empi_atom) * p->d_atoms.n, hipMemcpyDeviceToHost, s));
+    EMPI_HIP(hipMemcpyAsync(p->h_energies.data(), p->d_energies.p, sizeof(double) * p->d_energies.n,
+                            hipMemcpyDeviceToHost, s));
+    EMPI_HIP(hipMemcpyAsync(p->h_state.data(), p->d_state.p, sizeof(UnitState) * U, hipMemcpyDeviceToHost, s));
+    EMPI_HIP(hipMemcpyAsync(p->h_counter, p->d_counter.p, sizeof(unsigned long long) * 2 * kNumClasses,
+                            hipMemcpyDeviceToHost, s));
+    EMPI_HIP(hipStreamSynchronize(s));
+    if (p->profiling) {
+        empi_status pst = collect_profile(p);
+        if (pst != EMPI_OK) return pst;
+    }
+    p->has_results = true;
+    return EMPI_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- C ABI
+
+extern "C" {
+
+const char* empi_strerror(int status) {
+    switch (status) {
+        case EMPI_OK: return "ok";
+        case EMPI_SIGNAL_IS_EMPTY: return "signalIsEmpty";
+        case EMPI_ENERGY_MAP_IS_EMPTY: return "energyMapIsEmpty";
+        case EMPI_TOO_LONG: return "signalFileIsTooLongForThisDecomposition";
+        case EMPI_INVALID_ARGUMENT: return "invalidArgument";
+        case EMPI_NO_MEMORY: return "insufficientMemory";
+        case EMPI_DEVICE: return "deviceError";
+        case EMPI_UNSUPPORTED: return "unsupportedSize";
+        default: return "internalError";
+    }
+}
+
+const char* empi_version(void) { return "empi-mi355x 0.1 (gfx950)"; }
+
+empi_status empi_plan_create(const empi_item* items, int count, empi_plan** out) {
+    if (!out) return EMPI_INVALID_ARGUMENT;
+    *out = nullptr;
+    empi_plan* p = new (std::nothrow) empi_plan();
+    if (!p) return EMPI_NO_MEMORY;
+    empi_status st;
+    try {
+        st = plan_build(p, items, count, 0);
+    } catch (const std::bad_alloc&) {
+        st = EMPI_NO_MEMORY;
+    } catch (...) {
+        st = EMPI_INTERNAL;
+    }
+    if (st != EMPI_OK) {
+        delete p;
+        return st;
+    }
+    *out = p;
+    return EMPI_OK;
+}
+
+empi_status empi_plan_run(empi_plan* plan, const float* signal, int signal_on_device, void* stream) {
+    if (!plan || !signal) return EMPI_INVALID_ARGUMENT;
+    try {
+        return plan_run(plan, signal, signal_on_device, static_cast<hipStream_t>(stream));
+    } catch (const std::bad_alloc&) {
+        return EMPI_NO_MEMORY;
+    } catch (...) {
+        return EMPI_INTERNAL;
+    }
+}
+
+empi_status empi_plan_item_result(const empi_plan* p, int i, empi_atom* atoms, int* counts, double* energies) {
+    if (!p || i < 0 || i >= static_cast<int>(p->items.size())) return EMPI_INVALID_ARGUMENT;
+    if (!p->has_results) return EMPI_INVALID_ARGUMENT;
+    const ItemInfo& info = p->items[i];
+    const int C = info.item.channels, I = info.item.iterationMax;
+    empi_status status = EMPI_OK;
+    std::vector<int> cnt(C, 0);
+    if (info.item.variant == EMPI_SMP) {
+        for (int c = 0; c < C; ++c) {
+            const UnitState& s = p->h_state[info.units[c]];
+            cnt[c] = s.iteration;
+            if (s.status != EMPI_OK && status == EMPI_OK) status = s.status;
+        }
+    } else {
+        const UnitState& s = p->h_state[info.units[0]];
+        for (int c = 0; c < C; ++c) cnt[c] = s.iteration;
+        status = s.status;
+    }
+    if (counts) std::memcpy(counts, cnt.data(), sizeof(int) * C);
+    if (atoms) std::memcpy(atoms, p->h_atoms.data() + info.atom_off, sizeof(empi_atom) * C * I);
+    if (energies)
+        std::memcpy(energies, p->h_energies.data() + info.energy_off, sizeof(double) * info.rows * (I + 1));
+    return status;
+}
+
+size_t empi_plan_item_atom_count(const empi_plan* p, int i) {
+    if (!p || i < 0 || i >= static_cast<int>(p->items.size())) return 0;
+    return static_cast<size_t>(p->dicts[p->items[i].dict].entries);
+}
+
+double empi_plan_entries_evaluated(const empi_plan* p) {
+    if (!p) return 0.0;
+    double n = 0.0;
+    for (int c = 0; c < kNumClasses; ++c) n += static_cast<double>(p->h_counter[2 * c]);
+    return n;
+}
+
+empi_status empi_plan_set_profiling(empi_plan* p, int on) {
+    if (!p) return EMPI_INVALID_ARGUMENT;
+    p->profiling = on != 0;
+    return EMPI_OK;
+}
+
+empi_status empi_plan_kernel_stats(const empi_plan* p, int cls, int* sub_length, long long* launches,
+                                   double* total_ms, double* entries, double* entry_channels) {
+    if (!p || cls < 0 || cls >= kNumClasses) return EMPI_INVALID_ARGUMENT;
+    if (sub_length) *sub_length = 1 << (cls + kMinLgQ);
+    if (launches) *launches = p->class_launches[cls];
+    if (total_ms) *total_ms = p->class_ms[cls];
+    if (entries) *entries = static_cast<double>(p->h_counter[2 * cls]);
+    if (entry_channels) *entry_channels = static_cast<double>(p->h_counter[2 * cls + 1]);
+    return EMPI_OK;
+}
+
+int empi_plan_kernel_classes(void) { return kNumClasses; }
+
+void empi_plan_destroy(empi_plan* plan) { delete plan; }
+
+// ----------------------------------------------------------- workspace API
+
+empi_status empi_ws_create(const empi_dict_params* dict, double fs, int channels, int samples, int constraint,
+                           empi_ws** out) {
+    if (!out || !dict) return EMPI_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (channels <= 0 || samples < 2) return EMPI_INVALID_ARGUMENT;
+    empi_ws* w = new (std::nothrow) empi_ws();
+    if (!w) return EMPI_NO_MEMORY;
+    empi_item it{};
+    it.channels = channels;
+    it.samples = samples;
+    it.fs = fs;
+    // one unit over all channels; the constraint is carried by the variant
+    it.variant = channels == 1 ? (constraint ? EMPI_MMP1 : EMPI_MMP3) : (constraint ? EMPI_MMP1 : EMPI_MMP3);
+    it.iterationMax = 1;
+    it.energyPercent = 100.0;
+    it.dict = *dict;
+    empi_status st;
+    try {
+        st = plan_build(&w->plan, &it, 1, constraint ? 1 : 0);
+        if (st == EMPI_OK) st = w->d_one.alloc(1);
+    } catch (const std::bad_alloc&) {
+        st = EMPI_NO_MEMORY;
+    } catch (...) {
+        st = EMPI_INTERNAL;
+    }
+    if (st != EMPI_OK) {
+        delete w;
+        return st;
+    }
+    w->C = channels;
+    w->N = samples;
+    w->fs = fs;
+    *out = w;
+    return EMPI_OK;
+}
+
+empi_status empi_ws_compute(empi_ws* w, const double* signal) {
+    if (!w || !signal) return EMPI_INVALID_ARGUMENT;
+    empi_plan* p = &w->plan;
+    hipStream_t s = nullptr;
+    EMPI_HIP(hipMemcpyAsync(p->d_res.p, signal, sizeof(double) * w->C * w->N, hipMemcpyHostToDevice, s));
+    // state: active, energy only for the input scaling of the FP32 maps
+    double E = 0.0;
+    for (int c = 0; c < w->C; ++c) {
+        double sum = 0.0;
+        for (int n = 0; n < w->N; ++n) sum += signal[static_cast<size_t>(c) * w->N + n] * signal[static_cast<size_t>(c) * w->N + n];
+        E += sum / w->fs;
+    }
+    UnitState us{};
+    us.active = 1;
+    us.totalEnergy = us.residEnergy = E;
+    const double ms = E * w->fs / (static_cast<double>(w->C) * w->N);
+    int ex = 0;
+    if (ms > 0) std::frexp(std::sqrt(ms), &ex);
+    ex = std::max(-100, std::min(100, ex));
+    us.xscale = std::ldexp(1.0f, -ex);
+    EMPI_HIP(hipMemcpyAsync(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice, s));
+    EMPI_HIP(hipMemsetAsync(p->d_counter.p, 0, sizeof(unsigned long long) * 2 * kNumClasses, s));
+    empi_status st = launch_all_columns(p, s, 1);
+    if (st != EMPI_OK) return st;
+    EMPI_HIP(hipStreamSynchronize(s));
+    w->computed = true;
+    w->pending = false;
+    return EMPI_OK;
+}
+
+empi_status empi_ws_find_best(empi_ws* w, empi_atom* out) {
+    if (!w || !out) return EMPI_INVALID_ARGUMENT;
+    if (!w->computed) return EMPI_INTERNAL;   // "internalLogicError", gabor.cpp:305-310
+    empi_plan* p = &w->plan;
+    hipStream_t s = nullptr;
+    empi_status st;
+    if (w->pending) {
+        if ((st = launch_all_columns(p, s, 0)) != EMPI_OK) return st;
+        w->pending = false;
+    }
+    // the search writes iteration 0 of the unit's atom rows
+    UnitState us;
+    EMPI_HIP(hipMemcpyAsync(&us, p->d_state.p, sizeof us, hipMemcpyDeviceToHost, s));
+    EMPI_HIP(hipStreamSynchronize(s));
+    us.iteration = 0;
+    us.active = 1;
+    us.status = EMPI_OK;
+    EMPI_HIP(hipMemcpyAsync(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
+                       p->d_scales.p, p->d_cells.p, p->d_res.p, 1);
+    EMPI_HIP(hipGetLastError());
+    EMPI_HIP(hipMemcpyAsync(&us, p->d_state.p, sizeof us, hipMemcpyDeviceToHost, s));
+    const int I = p->items[0].item.iterationMax;
+    std::vector<empi_atom> rows(static_cast<size_t>(w->C) * I);
+    EMPI_HIP(hipMemcpyAsync(rows.data(), p->d_atoms.p, sizeof(empi_atom) * rows.size(), hipMemcpyDeviceToHost, s));
+    EMPI_HIP(hipStreamSynchronize(s));
+    if (us.status != EMPI_OK) {
+        us.status = EMPI_OK;
+        us.active = 1;
+        EMPI_HIP(hipMemcpy(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice));
+        return EMPI_ENERGY_MAP_IS_EMPTY;
+    }
+    for (int c = 0; c < w->C; ++c) out[c] = rows[static_cast<size_t>(c) * I];
+    return EMPI_OK;
+}
+
+size_t empi_ws_atom_count(const empi_ws* w) {
+    return w ? static_cast<size_t>(w->plan.dicts[0].entries) : 0;
+}
+
+empi_status empi_ws_subtract(empi_ws* w, const empi_atom* atom, int channel, double* residual) {
+    if (!w || !atom || !residual || channel < 0 || channel >= w->C) return EMPI_INVALID_ARGUMENT;
+    if (!w->computed) return EMPI_INTERNAL;
+    empi_plan* p = &w->plan;
+    hipStream_t s = nullptr;
+    double* dres = p->d_res.p + static_cast<size_t>(channel) * w->N;
+    // a pending re-evaluation for a different atom must run before this one moves tA/sA
+    if (w->pending) {
+        UnitState us;
+        EMPI_HIP(hipMemcpy(&us, p->d_state.p, sizeof us, hipMemcpyDeviceToHost));
+        const double tA = atom->params[2] / w->fs, sA = atom->params[3] / w->fs;
+        if (us.tA != tA || us.sA != sA) {
+            empi_status st = launch_all_columns(p, s, 0);
+            if (st != EMPI_OK) return st;
+            w->pending = false;
+        }
+    }
+    EMPI_HIP(hipMemcpyAsync(dres, residual, sizeof(double) * w->N, hipMemcpyHostToDevice, s));
+    EMPI_HIP(hipMemcpyAsync(w->d_one.p, atom, sizeof(empi_atom), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_subtract_one<kSelectThreads>, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p,
+                       p->d_state.p, w->d_one.p, channel, p->d_res.p);
+    EMPI_HIP(hipGetLastError());
+    EMPI_HIP(hipMemcpyAsync(residual, dres, sizeof(double) * w->N, hipMemcpyDeviceToHost, s));
+    EMPI_HIP(hipStreamSynchronize(s));
+    w->pending = true;
+    return EMPI_OK;
+}
+
+void empi_ws_destroy(empi_ws* w) { delete w; }
+
+empi_status empi_subtract_atom_from_signal(double fs, empi_atom* atom, double* x, int N, int fit) {
+    if (!atom || !x || N <= 0 || !(fs > 0)) return EMPI_INVALID_ARGUMENT;
+    // gabor.cpp:143-187 on the host
+    const double sA = atom->params[3] / fs;
+    const double fA = atom->params[4] * fs / 2.0;
+    const double tA = atom->params[2] / fs;
+    const double amplitude = fit ? 1.0 : atom->params[1];
+    const double omega = 2.0 * M_PI * fA;
+    const double phase = atom->params[5];
+    const long n = N;
+    const double hw = kGaussHalfWidth * sA;
+    long l = std::lrint((tA - hw) * fs), r = std::lrint((tA + hw) * fs);
+    const int iL = static_cast<int>(l > 0 ? l : 0);
+    const int iR = static_cast<int>(r < n - 1 ? r : n - 1);
+    const int Nw = 1 + iR - iL;
+    if (Nw <= 0) return EMPI_OK;
+    std::vector<double> w(static_cast<size_t>(Nw));
+    for (int i = 0; i < Nw; ++i) {
+        const double t = (i + iL) / fs - tA;
+        w[i] = amplitude * std::cos(omega * t + phase);
+    }
+    const double gw = std::sqrt(M_PI) / sA;
+    const double ga = gw * (1.0 / fs), gb = gw * (tA - iL / fs);
+    for (int i = 0; i < Nw; ++i) {
+        const double xw = ga * i - gb;
+        w[i] *= std::exp(-xw * xw);
+    }
+    if (fit) {
+        double norm2 = 0.0;
+        for (int i = 0; i < Nw; ++i) norm2 += w[i] * w[i];
+        const double norm = std::sqrt(norm2);
+        for (int i = 0; i < Nw; ++i) w[i] /= norm;
+        double modulus = 0.0;
+        for (int i = 0; i < Nw; ++i) modulus += w[i] * x[iL + i];
+        for (int i = 0; i < Nw; ++i) w[i] *= modulus;
+        atom->params[0] = modulus;
+        atom->params[1] = modulus / norm;
+    }
+    for (int i = 0; i < Nw; ++i) x[iL + i] -= w[i];
+    return EMPI_OK;
+}
+
+}  // extern "C"

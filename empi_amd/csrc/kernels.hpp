@@ -1,0 +1,72 @@
+// Device-side data layout shared by the host plan and the kernels.
+//
+// HBM layout of one batch (all arrays flat, offsets per unit):
+//   residual   FP64 [unit][channel][sample]        the MP residual
+//   cellbest   u64  [unit][scale][t][g]            best product of one cell
+//              (high 32 bits: FP32 product bits, low 32 bits: ~bin)
+//   atoms      empi_atom [item][channel][iteration]
+//   energies   FP64 [item][row][iteration + 1]
+// A "unit" is one MP loop: one channel of an SMP item, the channel sum of an
+// MMP2 item, or all channels of an MMP1/MMP3 item.  A "cell" is the part of
+// one map column (time index t) that one workgroup slot pair transforms:
+// sub-transforms k2 = {g, P-g} (g = 0 holds {0, P/2}).
+#pragma once
+
+#include <cstdint>
+
+#include "empi_hip.h"
+
+namespace empi {
+
+struct ScaleDev {
+    double s, fs, tMax;
+    long long cells_before;   // cell offset of this scale inside its unit
+    int N, Nfft, M, T, F, Q, P, G;
+    int lgQ, lgNfft;
+    float alpha;              // s*fs/Nfft: s*f4 = j*alpha
+    float Kp;                 // 2*sqrt(2)/(s*fs)
+    float gA;                 // pi/(s*fs)^2
+    int Wmax;
+};
+
+struct UnitDev {
+    long long res_off;        // residual offset (doubles)
+    long long cell_off;       // cellbest offset
+    long long in_off;         // input offset (floats) of the item's channel 0
+    int C;                    // channels of the unit
+    int N;
+    int in_channel;           // first input channel read by this unit
+    int in_sum;               // number of input channels summed into one (MMP2), else 0
+    int scale0, L;            // scales of its dictionary in the ScaleDev array
+    int constraint;           // 1 = MMP1
+    int iterMax;
+    double fs;
+    double residualEnergy;
+    empi_atom* atoms;         // first atom slot of the unit's channel 0
+    int atom_cstride;         // atoms between consecutive channels
+    int pad;
+    double* energies;         // [iterMax + 1]
+    double2* fitv;            // scratch: C complex values of the chosen column
+};
+
+struct UnitState {
+    int active;
+    int iteration;
+    int status;
+    int pad;
+    double totalEnergy;
+    double residEnergy;
+    double tA, sA;            // last subtracted atom, seconds
+    float xscale;             // power-of-two input scaling of the FP32 maps
+    int pad2;
+};
+
+struct Job {                  // one (unit, scale) pair in an FFT-size class
+    int unit;
+    int scale;                // global ScaleDev index
+    long long cell_off;       // absolute cellbest offset of this scale
+    int tile0;
+    int tiles;
+};
+
+}  // namespace empi

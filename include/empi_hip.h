@@ -1,0 +1,156 @@
+/*
+ * empi_hip.h -- C ABI of the MI355X Matching Pursuit engine.
+ *
+ * Plain C types only (no C++, no torch).  Two granularities:
+ *
+ *  1. empi_ws_*: the reference's Workspace/WorkspaceBuilder operator API
+ *     (inc/classes.hpp:65-80), one call per reference virtual, so a host can
+ *     run the reference's own Decomposition loop on top of it.
+ *  2. empi_plan_*: the whole decomposition loop (inc/classes.hpp:89-107,
+ *     src/classes.cpp:26-87, src/mmp.cpp:9-45) for a batch of signals of
+ *     any lengths and variants, device-resident from the full map build to
+ *     the last atom.
+ *
+ * Status codes map 1:1 onto the reference's exception strings
+ * (empi_strerror returns the camelCase text, src/empi.cpp:169-193).
+ * Handles are single-threaded; different handles may run concurrently.
+ */
+#ifndef EMPI_HIP_H
+#define EMPI_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int empi_status;
+
+enum {
+    EMPI_OK = 0,
+    EMPI_SIGNAL_IS_EMPTY = 1,        /* "signalIsEmpty"       src/classes.cpp:37-39 */
+    EMPI_ENERGY_MAP_IS_EMPTY = 2,    /* "energyMapIsEmpty"    src/gabor.cpp:108-110 */
+    EMPI_TOO_LONG = 3,               /* "signalFileIsTooLongForThisDecomposition" src/gabor.cpp:239-241 */
+    EMPI_INVALID_ARGUMENT = 4,       /* "invalidArgument" */
+    EMPI_NO_MEMORY = 5,              /* "insufficientMemory"  src/empi.cpp:189-190 */
+    EMPI_INTERNAL = 6,               /* "internalError"       src/empi.cpp:191-192 */
+    EMPI_DEVICE = 7,                 /* "deviceError": a HIP runtime call failed */
+    EMPI_UNSUPPORTED = 8             /* "unsupportedSize": a window beyond the kernels' limit */
+};
+
+/* Decomposition variants (src/empi.cpp:114-130). */
+enum { EMPI_SMP = 0, EMPI_MMP1 = 1, EMPI_MMP2 = 2, EMPI_MMP3 = 3 };
+
+/* Multichannel constraint of a workspace (inc/classes.hpp:63). */
+enum { EMPI_CONSTRAINT_NONE = 0, EMPI_CONSTRAINT_MMP1 = 1 };
+
+/* GaborWorkspaceBuilder(energyError, scaleMin, scaleMax, freqMax)
+ * (inc/gabor.hpp:67-78); use 0 / INFINITY / INFINITY for the defaults of
+ * src/empi.cpp:63-68. */
+typedef struct {
+    double energyError;
+    double scaleMin;
+    double scaleMax;
+    double freqMax;
+} empi_dict_params;
+
+/* Atom record: Atom{type=13, params[6]} (inc/base.hpp:24-29) plus the
+ * dictionary indices it was selected at. params = modulus, amplitude,
+ * position [samples], scale [samples], frequency [x Nyquist], phase. */
+typedef struct {
+    double params[6];
+    int scale;      /* scale index k */
+    int freq;       /* frequency bin */
+    int time;       /* time index */
+    int reserved;
+} empi_atom;
+
+/* ------------------------------------------------------------ workspace */
+
+typedef struct empi_ws empi_ws;
+
+/* GaborWorkspaceBuilder::prepareWorkspace (src/gabor.cpp:213-252). */
+empi_status empi_ws_create(const empi_dict_params* dict, double fs, int channels, int samples,
+                           int constraint, empi_ws** out);
+/* Workspace::compute: full map build of signal [channels][samples] FP64 (src/gabor.cpp:75-83). */
+empi_status empi_ws_compute(empi_ws* ws, const double* signal);
+/* Workspace::findBestMatch: one atom per channel into out[channels] (src/gabor.cpp:85-133). */
+empi_status empi_ws_find_best(empi_ws* ws, empi_atom* out);
+/* Workspace::getAtomCount: dictionary size sum F*T (src/gabor.cpp:135-141). */
+size_t      empi_ws_atom_count(const empi_ws* ws);
+/* Workspace::subtractAtom: subtract from residual[samples] (host FP64, updated
+ * in place) and re-evaluate the touched map region (src/gabor.cpp:189-209). */
+empi_status empi_ws_subtract(empi_ws* ws, const empi_atom* atom, int channel, double* residual);
+void        empi_ws_destroy(empi_ws* ws);
+
+/* Workspace::subtractAtomFromSignal (src/classes.cpp:15-22, src/gabor.cpp:143-187)
+ * on a host FP64 signal; fit != 0 refits params[0..1] by least squares. */
+empi_status empi_subtract_atom_from_signal(double fs, empi_atom* atom, double* signal,
+                                           int samples, int fit);
+
+/* ------------------------------------------------------------ batched plan */
+
+/* One signal of a batch: the settings of one reference run
+ * (src/empi.cpp:59-130, DecompositionSettings inc/classes.hpp:84-87). */
+typedef struct {
+    int channels;
+    int samples;
+    double fs;
+    int variant;              /* EMPI_SMP .. EMPI_MMP3 */
+    int iterationMax;
+    double energyPercent;     /* residualEnergy = 1 - 0.01 * energyPercent */
+    empi_dict_params dict;
+} empi_item;
+
+typedef struct empi_plan empi_plan;
+
+/* Builds every dictionary (shared between items with equal parameters),
+ * sizes and allocates all device buffers. */
+empi_status empi_plan_create(const empi_item* items, int count, empi_plan** out);
+
+/* Decomposes the batch.  signal: float32 samples of every item, item after
+ * item, each [channels][samples]; on the device when signal_on_device != 0,
+ * else host memory.  stream: a hipStream_t or NULL for the null stream.
+ * Returns after the work is complete. */
+empi_status empi_plan_run(empi_plan* plan, const float* signal, int signal_on_device, void* stream);
+
+/* Results of item i after empi_plan_run:
+ *   atoms    [channels][iterationMax]   (row c holds counts[c] atoms)
+ *   counts   [channels]
+ *   energies [rows][iterationMax + 1]   residual energy before each
+ *            iteration (entry 0 = total energy), then after the last
+ *            subtraction, NaN-padded; rows = channels for SMP, else 1.
+ * Any pointer may be NULL.  Returns the item's own status. */
+empi_status empi_plan_item_result(const empi_plan* plan, int item, empi_atom* atoms, int* counts,
+                                  double* energies);
+
+/* Dictionary size (sum F*T) of item i; 0 on a bad index. */
+size_t      empi_plan_item_atom_count(const empi_plan* plan, int item);
+
+/* Map entries (scale, bin, time) re-evaluated by the last run, full build
+ * included; each entry covers every channel of its unit. */
+double      empi_plan_entries_evaluated(const empi_plan* plan);
+
+/* Kernel accounting.  With profiling on, every launch of the fused column
+ * kernel of each sub-transform class is bracketed by HIP events on the
+ * launch stream; empi_plan_kernel_stats reports, for class cls of the last
+ * run: the class's sub-transform length, launches, summed kernel time and
+ * the map entries (and entry-channels) it re-evaluated. */
+empi_status empi_plan_set_profiling(empi_plan* plan, int on);
+int         empi_plan_kernel_classes(void);
+empi_status empi_plan_kernel_stats(const empi_plan* plan, int cls, int* sub_length, long long* launches,
+                                   double* total_ms, double* entries, double* entry_channels);
+
+void        empi_plan_destroy(empi_plan* plan);
+
+/* Camel-case reference error text for a status code. */
+const char* empi_strerror(int status);
+
+/* Library version string. */
+const char* empi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
