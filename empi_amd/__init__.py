@@ -27,7 +27,7 @@ STATUS_NAMES = {
 }
 
 EXPORTS = [
-    "empi_ws_create", "empi_ws_compute", "empi_ws_find_best", "empi_ws_atom_count",
+    "empi_dict_info", "empi_ws_create", "empi_ws_compute", "empi_ws_find_best", "empi_ws_atom_count",
     "empi_ws_subtract", "empi_ws_destroy", "empi_subtract_atom_from_signal",
     "empi_plan_create", "empi_plan_run", "empi_plan_item_result", "empi_plan_item_atom_count",
     "empi_plan_entries_evaluated", "empi_plan_set_profiling", "empi_plan_kernel_classes",
@@ -43,6 +43,11 @@ class DictParams(C.Structure):
 class Atom(C.Structure):
     _fields_ = [("params", C.c_double * 6), ("scale", C.c_int), ("freq", C.c_int),
                 ("time", C.c_int), ("reserved", C.c_int)]
+
+
+class ScaleInfo(C.Structure):
+    _fields_ = [("scale", C.c_double), ("Nfft", C.c_int), ("tCount", C.c_int),
+                ("fCount", C.c_int), ("reserved", C.c_int)]
 
 
 class Item(C.Structure):
@@ -70,6 +75,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         dp = C.POINTER(C.c_double)
+        L.empi_dict_info.argtypes = [C.POINTER(DictParams), C.c_double, C.c_int, C.POINTER(ScaleInfo),
+                                     C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
         L.empi_ws_create.argtypes = [C.POINTER(DictParams), C.c_double, C.c_int, C.c_int, C.c_int,
                                      C.POINTER(vp)]
         L.empi_ws_compute.argtypes = [vp, dp]
@@ -260,6 +267,19 @@ class Workspace:
             self.close()
         except Exception:
             pass
+
+
+def dictionary(fs, samples, energy_error, scale_min=0.0, scale_max=math.inf, freq_max=math.inf):
+    """Host-only dictionary description: (list of (s, Nfft, T, F), entries)."""
+    dp = dict_params(energy_error, scale_min, scale_max, freq_max)
+    n = C.c_int()
+    e = C.c_longlong()
+    _check(lib().empi_dict_info(C.byref(dp), float(fs), int(samples), None, 0, C.byref(n), C.byref(e)),
+           "empi_dict_info")
+    arr = (ScaleInfo * max(1, n.value))()
+    _check(lib().empi_dict_info(C.byref(dp), float(fs), int(samples), arr, n.value, C.byref(n), C.byref(e)),
+           "empi_dict_info")
+    return [(a.scale, a.Nfft, a.tCount, a.fCount) for a in arr[:n.value]], e.value
 
 
 def version():

@@ -102,7 +102,7 @@ constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 // 16 points per thread.
 template <int Q>
 struct Geo {
-    static constexpr int NSL = Q >= 2048 ? 2 : 4096 / Q;
+    static constexpr int NSL = Q >= 2048 ? 2 : (4096 / Q > 512 ? 512 : 4096 / Q);
     static constexpr int NPTS = NSL * Q;
     static constexpr int NT = NPTS / 16;
     static constexpr int NCELL = NSL / 2;
@@ -181,239 +181,370 @@ __device__ inline void cell_k2(int g, int P, int* ka, int* kb) {
 // ------------------------------------------------------------ column kernel
 
 struct CellInfo {
-    int valid;
+    int valid;      // re-evaluated by this pass
+    int exists;     // cell < T*G
     int iL;
     int W;
     int dint;
     float dfrac;
     int cell;
+    int ka, kb;     // sub-transforms of the cell's two slots, -1 = none
 };
 
-// product of one channel at one bin from the phase-corrected value u:
-// K' |u|^4 / ((1+E) Re(u)^2 + (1-E) Im(u)^2), without the constant K'
+// E = exp(-x) and 1 - E without cancellation, x = 2 pi (s f4)^2
+__device__ inline void gauss_factors(float x, float* E, float* omE) {
+    *E = __expf(-x);
+    // (1 - e^-x)/x by Taylor below 0.5 (truncation < 1e-8 relative)
+    float p = -1.0f / 40320.0f;
+    p = fmaf(p, x, 1.0f / 5040.0f);
+    p = fmaf(p, x, -1.0f / 720.0f);
+    p = fmaf(p, x, 1.0f / 120.0f);
+    p = fmaf(p, x, -1.0f / 24.0f);
+    p = fmaf(p, x, 1.0f / 6.0f);
+    p = fmaf(p, x, -0.5f);
+    p = fmaf(p, x, 1.0f);
+    *omE = x < 0.5f ? x * p : 1.0f - *E;
+}
+
+// product of one channel at one bin from the phase-corrected value u,
+// |u|^4 / ((1+E) Re(u)^2 + (1-E) Im(u)^2) (the constant K' applied later)
 __device__ inline float bin_term(float ur, float ui, float ja) {
-    const float xx = 6.283185307179586f * ja * ja;
-    const float E = __expf(-xx), omE = -expm1f(-xx);
+    float E, omE;
+    gauss_factors(6.283185307179586f * ja * ja, &E, &omE);
     const float a2 = ur * ur, b2 = ui * ui;
     const float num = a2 + b2, den = (1.0f + E) * a2 + omE * b2;
-    return den > 0.0f ? num * (num / den) : 0.0f;
+    return den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
+}
+
+// e^{2 pi i cyc}
+__device__ inline void twiddle_rev(float cyc, float* sn, float* cs) { sincospif(2.0f * cyc, sn, cs); }
+
+__device__ inline unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+    const unsigned lo = __shfl_xor(static_cast<unsigned>(v), m);
+    const unsigned hi = __shfl_xor(static_cast<unsigned>(v >> 32), m);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// Affected tiles of every job of one class (one workgroup per class):
+// columns with |t - tA| <= 3 (s + sA), widened by one column each side
+// (the column kernel applies the exact test per cell).
+constexpr int kPlanThreads = 1024;
+__global__ __launch_bounds__(kPlanThreads) void k_plan(const ClassDev* __restrict__ classes,
+                                                       const UnitState* __restrict__ state,
+                                                       const ScaleDev* __restrict__ scales, int full) {
+    __shared__ int part[kPlanThreads];
+    const ClassDev cd = classes[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int n = cd.njobs;
+    const int per = (n + kPlanThreads - 1) / kPlanThreads;
+    const int j0 = min(n, tid * per), j1 = min(n, j0 + per);
+    int sum = 0;
+    for (int j = j0; j < j1; ++j) {
+        const Job job = cd.jobs[j];
+        const UnitState st = state[job.unit];
+        int first = 0, cnt = 0;
+        if (st.active) {
+            if (full) {
+                cnt = job.tiles;
+            } else {
+                const ScaleDev sd = scales[job.scale];
+                const double R = (sd.s + st.sA) * kGaussHalfWidth;
+                const double scale = (sd.T - 1) / sd.tMax;
+                long lo = static_cast<long>(ceil((st.tA - R) * scale)) - 1;
+                long hi = static_cast<long>(floor((st.tA + R) * scale)) + 1;
+                lo = lo < 0 ? 0 : lo;
+                hi = hi > sd.T - 1 ? sd.T - 1 : hi;
+                if (lo <= hi) {
+                    const long c0 = lo * sd.G, c1 = hi * sd.G + sd.G - 1;
+                    first = static_cast<int>(c0 / cd.ncell);
+                    cnt = static_cast<int>(c1 / cd.ncell) - first + 1;
+                }
+            }
+        }
+        cd.first[j] = first;
+        cd.work[j] = cnt;
+        sum += cnt;
+    }
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < kPlanThreads; off <<= 1) {   // inclusive scan
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int j = j0; j < j1; ++j) {
+        const int c = cd.work[j];
+        cd.work[j] = run;
+        run += c;
+    }
+    if (tid == kPlanThreads - 1) cd.work[n] = part[tid];
 }
 
 // MULTI = false: one channel, no constraint (SMP units, MMP2 sums);
 // MULTI = true: channel sums of squares, MMP1 constraint when ud.constraint.
+// Persistent: workgroups stride through the class's affected tiles.
 template <int Q, bool MULTI>
 __global__ __launch_bounds__(Geo<Q>::NT) void k_columns(
-    const Job* __restrict__ jobs, int njobs, const ScaleDev* __restrict__ scales,
+    const ClassDev* __restrict__ classes, int cls, const ScaleDev* __restrict__ scales,
     const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
     const double* __restrict__ residual, unsigned long long* __restrict__ cellbest,
-    const float2* __restrict__ tw, int lgTWN, int full, unsigned long long* __restrict__ counter) {
+    unsigned long long* __restrict__ tilebest, const float2* __restrict__ tw, int lgTWN, int full,
+    unsigned long long* __restrict__ counter) {
     using G_ = Geo<Q>;
     constexpr int NT = G_::NT, NPTS = G_::NPTS, NCELL = G_::NCELL, EPT = G_::EPT;
     constexpr int NACC = MULTI ? EPT : 1;
+    constexpr int SEG = 2 * Q < 64 ? 2 * Q : 64;          // lanes sharing a cell
+    constexpr int GRP = 2 * Q >= NT ? 2 * Q / NT : 1;      // consecutive elements sharing a cell
+    static_assert(EPT % GRP == 0, "element groups");
     __shared__ float2 buf[Lds<NPTS>::SIZE];
     __shared__ CellInfo info[NCELL];
     __shared__ unsigned long long best[NCELL];
+    __shared__ unsigned long long tbest;
     __shared__ unsigned cnt_lds;
 
     const int tid = threadIdx.x;
-    // job lookup: last job with tile0 <= blockIdx.x
-    int lo = 0, hi = njobs - 1;
-    const int tileg = blockIdx.x;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile0 <= tileg) lo = mid; else hi = mid - 1;
-    }
-    const Job job = jobs[lo];
-    const UnitState st = state[job.unit];
-    if (!st.active) return;
-    const ScaleDev sd = scales[job.scale];
-    const UnitDev ud = units[job.unit];
-    const int cell0 = (tileg - job.tile0) * NCELL;
-    const int ncells = sd.T * sd.G;
-    const double s = sd.s, fs = sd.fs;
+    const int lane = tid & 63;
+    const ClassDev cd = classes[cls];
+    const int total = cd.work[cd.njobs];
+    unsigned long long evaluated = 0, evaluated_ch = 0;
 
-    for (int c = tid; c < NCELL; c += NT) {
-        CellInfo ci{};
-        ci.cell = cell0 + c;
-        ci.valid = ci.cell < ncells;
-        if (ci.valid) {
-            const int t = ci.cell / sd.G;
-            const double t0 = t * sd.tMax / (sd.T - 1);
-            if (!full) ci.valid = fabs(t0 - st.tA) <= (s + st.sA) * kGaussHalfWidth;
-            const double hw = kGaussHalfWidth * s;
-            long l = lrint((t0 - hw) * fs);
-            long r = lrint((t0 + hw) * fs);
-            ci.iL = static_cast<int>(l > 0 ? l : 0);
-            const int iR = static_cast<int>(r < sd.N - 1 ? r : sd.N - 1);
-            ci.W = min(iR - ci.iL + 1, 2 * Q);
-            const double t0fixed = t0 - ci.iL / fs;
-            const double d = t0fixed * fs;
-            const double dfl = floor(d);
-            ci.dint = static_cast<int>(dfl);
-            ci.dfrac = static_cast<float>(d - dfl);
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        // job: last j with work[j] <= w
+        int lo = 0, hi = cd.njobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cd.work[mid] <= w) lo = mid; else hi = mid - 1;
         }
-        info[c] = ci;
-        best[c] = 0ull;
-    }
-    if (tid == 0) cnt_lds = 0;
-    int any = 0;
-    for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
-    if (!__syncthreads_or(any)) return;
+        const Job job = cd.jobs[lo];
+        const int tile = cd.first[lo] + (w - cd.work[lo]);
+        const UnitState st = state[job.unit];
+        const ScaleDev sd = scales[job.scale];
+        const UnitDev ud = units[job.unit];
+        const int cell0 = tile * NCELL;
+        const int ncells = sd.T * sd.G;
+        const double s = sd.s, fs = sd.fs;
+        const int P = sd.P, G = sd.G, M = sd.M, F = sd.F, Nfft = sd.Nfft;
 
-    const int P = sd.P, G = sd.G, M = sd.M, F = sd.F, Nfft = sd.Nfft;
-    const int shM = lgTWN - (sd.lgNfft - 1);   // tw index shift for w_M
-    const int shN = lgTWN - sd.lgNfft;         // for w_Nfft
-    const float invN = 1.0f / static_cast<float>(Nfft);
-    const float xscale = st.xscale;
-    const bool mmp1 = MULTI && ud.constraint != 0;
-    const int C = MULTI ? ud.C : 1;
-
-    float acc0[NACC], acc1[NACC], acc2[NACC];
-    float accNyq = 0.0f;
-#pragma unroll
-    for (int i = 0; i < NACC; ++i) acc0[i] = acc1[i] = acc2[i] = 0.0f;
-    unsigned mycnt = 0;
-
-    for (int ch = 0; ch < C; ++ch) {
-        const double* x = residual + ud.res_off + static_cast<long long>(ch) * ud.N;
-        // load: packed pairs z[m] = w[2m] + i w[2m+1], pre-twiddled per slot
-#pragma unroll
-        for (int i = 0; i < EPT / 2; ++i) {
-            const int p = tid + i * NT;
-            const int c = p / Q, m = p % Q;
-            const CellInfo ci = info[c];
-            float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
-            if (ci.valid) {
-                const int n0 = 2 * m, n1 = 2 * m + 1;
-                float w0 = 0.f, w1 = 0.f;
-                if (n0 < ci.W) {
-                    const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
-                    w0 = static_cast<float>(x[ci.iL + n0] * xscale) * __expf(-sd.gA * u * u);
-                }
-                if (n1 < ci.W) {
-                    const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
-                    w1 = static_cast<float>(x[ci.iL + n1] * xscale) * __expf(-sd.gA * u * u);
-                }
-                const float2 z = make_float2(w0, w1);
-                int ka, kb;
-                cell_k2(ci.cell % G, P, &ka, &kb);
-                za = ka == 0 ? z : cmul(z, tw[(ka * m) << shM]);
-                if (kb >= 0) zb = cmul(z, tw[(kb * m) << shM]);
+        for (int c = tid; c < NCELL; c += NT) {
+            CellInfo ci{};
+            ci.cell = cell0 + c;
+            ci.exists = ci.cell < ncells;
+            ci.valid = ci.exists;
+            ci.kb = -1;
+            if (ci.exists) {
+                const int t = ci.cell / G;
+                const int g = ci.cell - t * G;
+                const double t0 = t * sd.tMax / (sd.T - 1);
+                if (!full) ci.valid = fabs(t0 - st.tA) <= (s + st.sA) * kGaussHalfWidth;
+                const double hw = kGaussHalfWidth * s;
+                const long l = lrint((t0 - hw) * fs);
+                const long r = lrint((t0 + hw) * fs);
+                ci.iL = static_cast<int>(l > 0 ? l : 0);
+                const int iR = static_cast<int>(r < sd.N - 1 ? r : sd.N - 1);
+                ci.W = min(iR - ci.iL + 1, 2 * Q);
+                const double d = (t0 - ci.iL / fs) * fs;
+                const double dfl = floor(d);
+                ci.dint = static_cast<int>(dfl);
+                ci.dfrac = static_cast<float>(d - dfl);
+                cell_k2(g, P, &ci.ka, &ci.kb);
             }
-            buf[lpad((2 * c) * Q + m)] = za;
-            buf[lpad((2 * c + 1) * Q + m)] = zb;
+            info[c] = ci;
+            best[c] = 0ull;
         }
-        __syncthreads();
-        fft_passes<Q, NT, NPTS, 1>(buf, tw, lgTWN);
+        if (tid == 0) {
+            cnt_lds = 0;
+            tbest = 0ull;
+        }
+        int any = 0;
+        for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
+        if (!__syncthreads_or(any)) continue;
 
-        // post: split the packed spectrum, phase-correct, product per bin
+        const int shM = lgTWN - (sd.lgNfft - 1);   // tw index shift for w_M
+        const int shN = lgTWN - sd.lgNfft;         // for w_Nfft
+        const float invN = 1.0f / static_cast<float>(Nfft);
+        const float xscale = st.xscale;
+        const bool mmp1 = MULTI && ud.constraint != 0;
+        const int C = MULTI ? ud.C : 1;
+
+        float acc0[NACC], acc1[NACC], acc2[NACC];
+        float accNyq = 0.0f;
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int e = tid + i * NT;
-            const int c = e / (2 * Q), rem = e % (2 * Q);
-            const int half = rem / Q, k1 = rem % Q;
-            const CellInfo ci = info[c];
-            if (!ci.valid) continue;
-            int ka, kb;
-            cell_k2(ci.cell % G, P, &ka, &kb);
-            const int k2 = half ? kb : ka;
-            if (k2 < 0) continue;
-            const int k = P * k1 + k2;
-            if (k >= F) continue;
-            const int slot = 2 * c + half;
-            const float2 Zk = buf[lpad(slot * Q + k1)];
-            float2 Zp;
-            if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
-            else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
-            const float2 A = make_float2(Zk.x + Zp.x, Zk.y - Zp.y);
-            const float2 B = make_float2(Zk.x - Zp.x, Zk.y + Zp.y);
-            const float2 w = tw[k << shN];
-            const float pr = w.x * B.x - w.y * B.y, pi = w.x * B.y + w.y * B.x;
-            const float Xr = 0.5f * (A.x + pi), Xi = 0.5f * (A.y - pr);
-            const bool high = 4 * k > Nfft;
-            const int j = high ? M - k : k;
-            const unsigned ci_int = (static_cast<unsigned>(j) * static_cast<unsigned>(ci.dint)) & static_cast<unsigned>(Nfft - 1);
-            float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
-            if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
-            float sn, cs;
-            sincospif(2.0f * cyc, &sn, &cs);
-            const float ur = Xr * cs - Xi * sn, ui = Xr * sn + Xi * cs;
-            if constexpr (!MULTI) {
-                ++mycnt;
-                atomicMax(&best[c], pack_best(sd.Kp * bin_term(ur, ui, static_cast<float>(j) * sd.alpha),
-                                              static_cast<unsigned>(k)));
-            } else if (mmp1 && !high) {
-                acc0[i] += ur * ur;
-                acc1[i] += ui * ui;
-                acc2[i] += ur * ui;
-            } else {
-                acc0[i] += bin_term(ur, ui, static_cast<float>(j) * sd.alpha);
+        for (int i = 0; i < NACC; ++i) acc0[i] = acc1[i] = acc2[i] = 0.0f;
+        unsigned mycnt = 0;
+
+        for (int ch = 0; ch < C; ++ch) {
+            const double* x = residual + ud.res_off + static_cast<long long>(ch) * ud.N;
+            // load: packed pairs z[m] = w[2m] + i w[2m+1], pre-twiddled per slot
+#pragma unroll
+            for (int i = 0; i < EPT / 2; ++i) {
+                const int p = tid + i * NT;
+                const int c = p / Q, m = p % Q;
+                const CellInfo ci = info[c];
+                float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
+                if (ci.valid) {
+                    const int n0 = 2 * m, n1 = 2 * m + 1;
+                    float w0 = 0.f, w1 = 0.f;
+                    if (n0 < ci.W) {
+                        const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
+                        w0 = static_cast<float>(x[ci.iL + n0] * xscale) * __expf(-sd.gA * u * u);
+                    }
+                    if (n1 < ci.W) {
+                        const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
+                        w1 = static_cast<float>(x[ci.iL + n1] * xscale) * __expf(-sd.gA * u * u);
+                    }
+                    const float2 z = make_float2(w0, w1);
+                    za = ci.ka == 0 ? z : cmul(z, tw[(ci.ka * m) << shM]);
+                    if (ci.kb >= 0) zb = cmul(z, tw[(ci.kb * m) << shM]);
+                }
+                buf[lpad((2 * c) * Q + m)] = za;
+                buf[lpad((2 * c + 1) * Q + m)] = zb;
+            }
+            __syncthreads();
+            fft_passes<Q, NT, NPTS, 1>(buf, tw, lgTWN);
+
+            // post: split the packed spectrum, phase-correct, product per bin
+            unsigned long long run = 0ull;
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) {
+                const int e = tid + i * NT;
+                const int c = e / (2 * Q), rem = e % (2 * Q);
+                const int half = rem / Q, k1 = rem % Q;
+                const CellInfo& ci = info[c];
+                const int k2 = half ? ci.kb : ci.ka;
+                const int k = P * k1 + k2;
+                if (ci.valid && k2 >= 0 && k < F) {
+                    const int slot = 2 * c + half;
+                    const float2 Zk = buf[lpad(slot * Q + k1)];
+                    float2 Zp;
+                    if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
+                    else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
+                    const float2 A = make_float2(Zk.x + Zp.x, Zk.y - Zp.y);
+                    const float2 B = make_float2(Zk.x - Zp.x, Zk.y + Zp.y);
+                    const float2 wv = tw[k << shN];
+                    const float pr = wv.x * B.x - wv.y * B.y, pi = wv.x * B.y + wv.y * B.x;
+                    const float Xr = 0.5f * (A.x + pi), Xi = 0.5f * (A.y - pr);
+                    const bool high = 4 * k > Nfft;
+                    const int j = high ? M - k : k;
+                    const unsigned ci_int =
+                        (static_cast<unsigned>(j) * static_cast<unsigned>(ci.dint)) & static_cast<unsigned>(Nfft - 1);
+                    float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
+                    if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
+                    float sn, cs;
+                    twiddle_rev(cyc, &sn, &cs);
+                    const float ur = Xr * cs - Xi * sn, ui = Xr * sn + Xi * cs;
+                    if constexpr (!MULTI) {
+                        ++mycnt;
+                        run = max(run, pack_best(sd.Kp * bin_term(ur, ui, static_cast<float>(j) * sd.alpha),
+                                                 static_cast<unsigned>(k)));
+                    } else if (mmp1 && !high) {
+                        acc0[i] += ur * ur;
+                        acc1[i] += ui * ui;
+                        acc2[i] += ur * ui;
+                    } else {
+                        acc0[i] += bin_term(ur, ui, static_cast<float>(j) * sd.alpha);
+                    }
+                }
+                if constexpr (!MULTI) {
+                    if (i % GRP == GRP - 1) {
+#pragma unroll
+                        for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
+                        if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
+                        run = 0ull;
+                    }
+                }
+            }
+            // Nyquist bin k = M of g = 0 cells: X[M] = Re Z0 - Im Z0, u real
+            if (tid < NCELL && F > M) {
+                const CellInfo& ci = info[tid];
+                if (ci.valid && ci.ka == 0) {
+                    const float2 Z0 = buf[lpad((2 * tid) * Q)];
+                    const float a = Z0.x - Z0.y;
+                    accNyq += 0.5f * a * a;   // |u|^4/den with E = 1: a^4 / (2 a^2)
+                }
+            }
+            __syncthreads();
+        }
+
+        // final products of multi-channel units and per-cell maximum
+        if constexpr (MULTI) {
+            unsigned long long run = 0ull;
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) {
+                const int e = tid + i * NT;
+                const int c = e / (2 * Q), rem = e % (2 * Q);
+                const int half = rem / Q, k1 = rem % Q;
+                const CellInfo& ci = info[c];
+                const int k2 = half ? ci.kb : ci.ka;
+                const int k = P * k1 + k2;
+                if (ci.valid && k2 >= 0 && k < F) {
+                    ++mycnt;
+                    const bool high = 4 * k > Nfft;
+                    float p;
+                    if (mmp1 && !high) {
+                        const float Srr = acc0[i], Sii = acc1[i], Sri = acc2[i];
+                        const float Dr = Srr - Sii, Di = 2.0f * Sri;
+                        const float Dm = hypotf(Dr, Di);
+                        const float lam = 0.5f * (Srr + Sii + Dm);
+                        float cos2 = 1.0f;
+                        if (Dm > 0.0f) cos2 = Dr >= 0.0f ? (Dm + Dr) / (2.0f * Dm) : (Di * Di) / (2.0f * Dm * (Dm - Dr));
+                        const float ja = static_cast<float>(k) * sd.alpha;
+                        float E, omE;
+                        gauss_factors(6.283185307179586f * ja * ja, &E, &omE);
+                        const float den = omE + 2.0f * E * cos2;
+                        p = den > 0.0f ? sd.Kp * lam / den : 0.0f;
+                    } else {
+                        p = sd.Kp * acc0[i];
+                    }
+                    run = max(run, pack_best(p, static_cast<unsigned>(k)));
+                }
+                if (i % GRP == GRP - 1) {
+#pragma unroll
+                    for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
+                    if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
+                    run = 0ull;
+                }
             }
         }
-        // Nyquist bin k = M of g = 0 cells: X[M] = Re Z0 - Im Z0, u real
         if (tid < NCELL && F > M) {
-            const CellInfo ci = info[tid];
-            if (ci.valid && ci.cell % G == 0) {
-                const float2 Z0 = buf[lpad((2 * tid) * Q)];
-                const float a = Z0.x - Z0.y;
-                accNyq += 0.5f * a * a;   // num^2/den with E = 1: a^4 / (2 a^2)
+            const CellInfo& ci = info[tid];
+            if (ci.valid && ci.ka == 0) {
+                ++mycnt;
+                atomicMax(&best[tid], pack_best(sd.Kp * accNyq, static_cast<unsigned>(M)));
             }
+        }
+        if (mycnt) atomicAdd(&cnt_lds, mycnt);
+        __syncthreads();
+        // cell bests out, tile best over re-evaluated and untouched cells
+        unsigned long long tb = 0ull;
+        for (int c = tid; c < NCELL; c += NT) {
+            const CellInfo& ci = info[c];
+            if (!ci.exists) continue;
+            unsigned long long cb;
+            if (ci.valid) {
+                cb = best[c];
+                cellbest[job.cell_off + ci.cell] = cb;
+            } else {
+                cb = cellbest[job.cell_off + ci.cell];
+            }
+            const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(cb);
+            const unsigned key = (bin << 10) | static_cast<unsigned>(c);
+            tb = max(tb, (cb & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFu - key));
+        }
+        if (tb) atomicMax(&tbest, tb);
+        __syncthreads();
+        if (tid == 0) {
+            tilebest[job.tile_off + tile] = tbest;
+            evaluated += cnt_lds;
+            evaluated_ch += static_cast<unsigned long long>(cnt_lds) * (MULTI ? ud.C : 1);
         }
         __syncthreads();
     }
-
-    // final products and per-cell maximum
-    if constexpr (MULTI) {
-#pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int e = tid + i * NT;
-            const int c = e / (2 * Q), rem = e % (2 * Q);
-            const int half = rem / Q, k1 = rem % Q;
-            const CellInfo ci = info[c];
-            if (!ci.valid) continue;
-            int ka, kb;
-            cell_k2(ci.cell % G, P, &ka, &kb);
-            const int k2 = half ? kb : ka;
-            if (k2 < 0) continue;
-            const int k = P * k1 + k2;
-            if (k >= F) continue;
-            ++mycnt;
-            const bool high = 4 * k > Nfft;
-            float p;
-            if (mmp1 && !high) {
-                const float Srr = acc0[i], Sii = acc1[i], Sri = acc2[i];
-                const float Dr = Srr - Sii, Di = 2.0f * Sri;
-                const float Dm = hypotf(Dr, Di);
-                const float lam = 0.5f * (Srr + Sii + Dm);
-                float cos2 = 1.0f;
-                if (Dm > 0.0f) cos2 = Dr >= 0.0f ? (Dm + Dr) / (2.0f * Dm) : (Di * Di) / (2.0f * Dm * (Dm - Dr));
-                const float ja = static_cast<float>(k) * sd.alpha;
-                const float xx = 6.283185307179586f * ja * ja;
-                const float E = __expf(-xx), omE = -expm1f(-xx);
-                const float den = omE + 2.0f * E * cos2;
-                p = den > 0.0f ? sd.Kp * lam / den : 0.0f;
-            } else {
-                p = sd.Kp * acc0[i];
-            }
-            atomicMax(&best[c], pack_best(p, static_cast<unsigned>(k)));
-        }
-    }
-    if (tid < NCELL && F > M) {
-        const CellInfo ci = info[tid];
-        if (ci.valid && ci.cell % G == 0) {
-            ++mycnt;
-            atomicMax(&best[tid], pack_best(sd.Kp * accNyq, static_cast<unsigned>(M)));
-        }
-    }
-    if (mycnt) atomicAdd(&cnt_lds, mycnt);
-    __syncthreads();
-    for (int c = tid; c < NCELL; c += NT) {
-        if (info[c].valid) cellbest[job.cell_off + info[c].cell] = best[c];
-    }
-    if (tid == 0 && counter) {
-        constexpr int cls = ilog2c(Q) - 1;
-        atomicAdd(&counter[2 * cls], static_cast<unsigned long long>(cnt_lds));
-        atomicAdd(&counter[2 * cls + 1], static_cast<unsigned long long>(cnt_lds) * ud.C);
+    if (tid == 0 && counter && evaluated) {
+        constexpr int cls_ix = ilog2c(Q) - 1;
+        atomicAdd(&counter[2 * cls_ix], evaluated);
+        atomicAdd(&counter[2 * cls_ix + 1], evaluated_ch);
     }
 }
 
@@ -621,7 +752,8 @@ __device__ inline bool sel_better(const SelBest& a, const SelBest& b) {
 template <int NT>
 __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units, UnitState* __restrict__ state,
                                                const ScaleDev* __restrict__ scales,
-                                               const unsigned long long* __restrict__ cellbest,
+                                               const UnitScale* __restrict__ uscales,
+                                               const unsigned long long* __restrict__ tilebest,
                                                double* __restrict__ residual, int mode) {
     __shared__ double red[NT];
     __shared__ float redp[NT];
@@ -634,22 +766,22 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
     const UnitDev ud = units[u];
     const int it = st.iteration;
 
-    // 1. argmax over the cell bests: product desc, then scale, bin, time asc
+    // 1. argmax over the tile bests: product desc, then scale, bin, time asc
     SelBest b{-1.0f, ~0ull};
-    long long base = ud.cell_off;
     for (int l = 0; l < ud.L; ++l) {
-        const ScaleDev& sd = scales[ud.scale0 + l];
-        const int n = sd.T * sd.G;
-        for (int i = tid; i < n; i += NT) {
-            const unsigned long long v = cellbest[base + i];
+        const UnitScale us = uscales[ud.us_off + l];
+        const int G = scales[ud.scale0 + l].G;
+        for (int i = tid; i < us.tiles; i += NT) {
+            const unsigned long long v = tilebest[us.tile_off + i];
             SelBest c;
             c.p = __uint_as_float(static_cast<unsigned>(v >> 32));
-            const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(v & 0xFFFFFFFFu);
+            const unsigned key = 0xFFFFFFFFu - static_cast<unsigned>(v & 0xFFFFFFFFu);
+            const unsigned bin = key >> 10;
+            const int cell = i * us.ncell + static_cast<int>(key & 1023u);
             c.key = (static_cast<unsigned long long>(l) << 52) | (static_cast<unsigned long long>(bin) << 26) |
-                    static_cast<unsigned long long>(i / sd.G);
+                    static_cast<unsigned long long>(cell / G);
             if (sel_better(c, b)) b = c;
         }
-        base += n;
     }
     redp[tid] = b.p;
     redk[tid] = b.key;
@@ -827,8 +959,11 @@ struct DevBuf {
 
 struct ClassJobs {
     std::vector<Job> jobs;
-    int tiles = 0;
+    long long tiles = 0;       // all tiles of the class (full build)
+    int slot = -1;             // index in the device ClassDev array
     DevBuf<Job> d;
+    DevBuf<int> work;          // [njobs + 1]
+    DevBuf<int> first;         // [njobs]
 };
 
 struct ItemInfo {
@@ -845,18 +980,25 @@ struct ItemInfo {
 
 constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
 
-typedef void (*ColumnLauncher)(int tiles, hipStream_t s, const Job* jobs, int njobs, const ScaleDev* scales,
+typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
                                const UnitDev* units, const UnitState* state, const double* residual,
-                               unsigned long long* cellbest, const float2* tw, int lgTWN, int full,
-                               unsigned long long* counter);
+                               unsigned long long* cellbest, unsigned long long* tilebest, const float2* tw,
+                               int lgTWN, int full, unsigned long long* counter);
 
 template <int Q, bool MULTI>
-void launch_columns(int tiles, hipStream_t s, const Job* jobs, int njobs, const ScaleDev* scales,
+void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
                     const UnitDev* units, const UnitState* state, const double* residual,
-                    unsigned long long* cellbest, const float2* tw, int lgTWN, int full,
-                    unsigned long long* counter) {
-    hipLaunchKernelGGL((k_columns<Q, MULTI>), dim3(tiles), dim3(Geo<Q>::NT), 0, s, jobs, njobs, scales, units,
-                       state, residual, cellbest, tw, lgTWN, full, counter);
+                    unsigned long long* cellbest, unsigned long long* tilebest, const float2* tw, int lgTWN,
+                    int full, unsigned long long* counter) {
+    hipLaunchKernelGGL((k_columns<Q, MULTI>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
+                       state, residual, cellbest, tilebest, tw, lgTWN, full, counter);
+}
+
+// persistent grid: enough workgroups to fill 256 CUs a few times over
+int column_grid(int Q, long long tiles) {
+    const int nt = Q >= 2048 ? (Q * 2) / 16 : 256;
+    const long long cap = nt >= 1024 ? 512 : nt >= 512 ? 1024 : 2048;
+    return static_cast<int>(std::max(1ll, std::min(tiles, cap)));
 }
 
 // [multi][class]
@@ -871,7 +1013,7 @@ const ColumnLauncher kLaunchers[2][kNumClasses] = {
      launch_columns<8192, true>}};
 
 int cells_per_wg(int Q) {
-    const int nsl = Q >= 2048 ? 2 : 4096 / Q;
+    const int nsl = Q >= 2048 ? 2 : std::min(512, 4096 / Q);
     return nsl / 2;
 }
 
@@ -893,6 +1035,11 @@ struct empi_plan {
     DevBuf<UnitState> d_state;
     DevBuf<double> d_res;
     DevBuf<unsigned long long> d_cells;
+    DevBuf<unsigned long long> d_tiles;
+    DevBuf<UnitScale> d_uscales;
+    DevBuf<ClassDev> d_classes;
+    std::vector<UnitScale> uscales_h;
+    int nclass_dev = 0;
     DevBuf<float2> d_tw;
     DevBuf<empi_atom> d_atoms;
     DevBuf<double> d_energies;
@@ -1075,11 +1222,13 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     p->twN = maxNfft;
     p->lgTWN = lg2(maxNfft);
 
-    // jobs per class
+    // jobs per class, tile bests per (unit, scale)
+    long long tile_off = 0;
     for (size_t ui = 0; ui < p->units.size(); ++ui) {
         UnitDev& u = p->units[ui];
         const int did = u.scale0;
         u.scale0 = p->dict_scale0[did];
+        u.us_off = static_cast<long long>(p->uscales_h.size());
         for (int l = 0; l < u.L; ++l) {
             const ScaleDev& s = p->scales_h[u.scale0 + l];
             const int cls = s.lgQ - kMinLgQ;
@@ -1091,10 +1240,16 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             j.cell_off = u.cell_off + s.cells_before;
             const int cpw = cells_per_wg(s.Q);
             j.tiles = static_cast<int>((static_cast<long long>(s.T) * s.G + cpw - 1) / cpw);
-            j.tile0 = cj.tiles;
-            if (static_cast<long long>(cj.tiles) + j.tiles > INT_MAX) return EMPI_UNSUPPORTED;
+            j.tile_off = tile_off;
+            tile_off += j.tiles;
+            if (cj.tiles + j.tiles > INT_MAX) return EMPI_UNSUPPORTED;
             cj.tiles += j.tiles;
             cj.jobs.push_back(j);
+            UnitScale us{};
+            us.tile_off = j.tile_off;
+            us.tiles = j.tiles;
+            us.ncell = cpw;
+            p->uscales_h.push_back(us);
         }
     }
 
@@ -1105,6 +1260,8 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_state.alloc(p->units.size())) != EMPI_OK) return st;
     if ((st = p->d_res.alloc(static_cast<size_t>(res_off))) != EMPI_OK) return st;
     if ((st = p->d_cells.alloc(static_cast<size_t>(cell_off))) != EMPI_OK) return st;
+    if ((st = p->d_tiles.alloc(static_cast<size_t>(tile_off))) != EMPI_OK) return st;
+    if ((st = p->d_uscales.alloc(p->uscales_h.size())) != EMPI_OK) return st;
     if ((st = p->d_tw.alloc(static_cast<size_t>(p->twN))) != EMPI_OK) return st;
     if ((st = p->d_atoms.alloc(static_cast<size_t>(atom_off))) != EMPI_OK) return st;
     if ((st = p->d_energies.alloc(static_cast<size_t>(energy_off))) != EMPI_OK) return st;
@@ -1126,18 +1283,37 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_scales.upload(p->scales_h.data(), p->scales_h.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_units.upload(p->units.data(), p->units.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_tw.upload(tw.data(), tw.size(), s0)) != EMPI_OK) return st;
-    for (auto& row : p->classes) {
-        for (ClassJobs& cj : row) {
+    if ((st = p->d_uscales.upload(p->uscales_h.data(), p->uscales_h.size(), s0)) != EMPI_OK) return st;
+    std::vector<ClassDev> cds;
+    for (int multi = 0; multi < 2; ++multi) {
+        for (int c = 0; c < kNumClasses; ++c) {
+            ClassJobs& cj = p->classes[multi][c];
             if (cj.jobs.empty()) continue;
             if ((st = cj.d.alloc(cj.jobs.size())) != EMPI_OK) return st;
             if ((st = cj.d.upload(cj.jobs.data(), cj.jobs.size(), s0)) != EMPI_OK) return st;
+            if ((st = cj.work.alloc(cj.jobs.size() + 1)) != EMPI_OK) return st;
+            if ((st = cj.first.alloc(cj.jobs.size())) != EMPI_OK) return st;
+            ClassDev cd{};
+            cd.jobs = cj.d.p;
+            cd.work = cj.work.p;
+            cd.first = cj.first.p;
+            cd.njobs = static_cast<int>(cj.jobs.size());
+            cd.ncell = cells_per_wg(1 << (c + kMinLgQ));
+            cj.slot = static_cast<int>(cds.size());
+            cds.push_back(cd);
         }
     }
+    p->nclass_dev = static_cast<int>(cds.size());
+    if ((st = p->d_classes.alloc(cds.size())) != EMPI_OK) return st;
+    if ((st = p->d_classes.upload(cds.data(), cds.size(), s0)) != EMPI_OK) return st;
     EMPI_HIP(hipStreamSynchronize(s0));
     return EMPI_OK;
 }
 
 empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
+    if (p->nclass_dev == 0) return EMPI_OK;
+    hipLaunchKernelGGL(k_plan, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p, p->d_state.p,
+                       p->d_scales.p, full);
     for (int mc = 0; mc < 2 * kNumClasses; ++mc) {
         const int multi = mc / kNumClasses, c = mc % kNumClasses;
         ClassJobs& cj = p->classes[multi][c];
@@ -1153,8 +1329,9 @@ empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
             evp = &p->ev[c][p->ev_used[c]++];
             EMPI_HIP(hipEventRecord(evp->first, s));
         }
-        kLaunchers[multi][c](cj.tiles, s, cj.d.p, static_cast<int>(cj.jobs.size()), p->d_scales.p, p->d_units.p,
-                      p->d_state.p, p->d_res.p, p->d_cells.p, p->d_tw.p, p->lgTWN, full, p->d_counter.p);
+        kLaunchers[multi][c](column_grid(1 << (c + kMinLgQ), cj.tiles), s, p->d_classes.p, cj.slot, p->d_scales.p,
+                             p->d_units.p, p->d_state.p, p->d_res.p, p->d_cells.p, p->d_tiles.p, p->d_tw.p,
+                             p->lgTWN, full, p->d_counter.p);
         if (evp) EMPI_HIP(hipEventRecord(evp->second, s));
     }
     EMPI_HIP(hipGetLastError());
@@ -1201,7 +1378,7 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     std::vector<UnitState> hs(U);
     for (int it = 0; it < p->maxIter; ++it) {
         hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
-                           p->d_state.p, p->d_scales.p, p->d_cells.p, p->d_res.p, 3);
+                           p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, 3);
         EMPI_HIP(hipGetLastError());
         if (it + 1 < p->maxIter) {
             if ((st = launch_all_columns(p, s, 0)) != EMPI_OK) return st;
@@ -1264,6 +1441,30 @@ const char* empi_strerror(int status) {
 }
 
 const char* empi_version(void) { return "empi-mi355x 0.1 (gfx950)"; }
+
+empi_status empi_dict_info(const empi_dict_params* dict, double fs, int samples, empi_scale_info* scales,
+                           int max_scales, int* count, long long* entries) {
+    if (!dict) return EMPI_INVALID_ARGUMENT;
+    Dictionary d;
+    empi_status st;
+    try {
+        st = build_dictionary(*dict, fs, samples, &d);
+    } catch (const std::bad_alloc&) {
+        return EMPI_NO_MEMORY;
+    }
+    if (st != EMPI_OK) return st;
+    const int L = static_cast<int>(d.scales.size());
+    for (int i = 0; i < L && i < max_scales && scales; ++i) {
+        scales[i].scale = d.scales[i].s;
+        scales[i].Nfft = d.scales[i].Nfft;
+        scales[i].tCount = d.scales[i].T;
+        scales[i].fCount = d.scales[i].F;
+        scales[i].reserved = 0;
+    }
+    if (count) *count = L;
+    if (entries) *entries = d.entries;
+    return EMPI_OK;
+}
 
 empi_status empi_plan_create(const empi_item* items, int count, empi_plan** out) {
     if (!out) return EMPI_INVALID_ARGUMENT;
@@ -1442,7 +1643,7 @@ empi_status empi_ws_find_best(empi_ws* w, empi_atom* out) {
     us.status = EMPI_OK;
     EMPI_HIP(hipMemcpyAsync(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
-                       p->d_scales.p, p->d_cells.p, p->d_res.p, 1);
+                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, 1);
     EMPI_HIP(hipGetLastError());
     EMPI_HIP(hipMemcpyAsync(&us, p->d_state.p, sizeof us, hipMemcpyDeviceToHost, s));
     const int I = p->items[0].item.iterationMax;

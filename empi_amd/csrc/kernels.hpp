@@ -4,6 +4,8 @@
 //   residual   FP64 [unit][channel][sample]        the MP residual
 //   cellbest   u64  [unit][scale][t][g]            best product of one cell
 //              (high 32 bits: FP32 product bits, low 32 bits: ~bin)
+//   tilebest   u64  [unit][scale][tile]            best of a workgroup's cells
+//              (high: product bits, low: ~(bin << 10 | cell in tile))
 //   atoms      empi_atom [item][channel][iteration]
 //   energies   FP64 [item][row][iteration + 1]
 // A "unit" is one MP loop: one channel of an SMP item, the channel sum of an
@@ -47,6 +49,7 @@ struct UnitDev {
     int pad;
     double* energies;         // [iterMax + 1]
     double2* fitv;            // scratch: C complex values of the chosen column
+    long long us_off;         // first UnitScale of the unit
 };
 
 struct UnitState {
@@ -61,12 +64,31 @@ struct UnitState {
     int pad2;
 };
 
-struct Job {                  // one (unit, scale) pair in an FFT-size class
+struct Job {                  // one (unit, scale) pair in a kernel class
     int unit;
     int scale;                // global ScaleDev index
     long long cell_off;       // absolute cellbest offset of this scale
-    int tile0;
+    long long tile_off;       // absolute tilebest offset of this scale
+    int tiles;                // ceil(T*G / cells per workgroup)
+    int pad;
+};
+
+// Work list of one kernel class, rebuilt every iteration by k_plan:
+// work[j] = affected tiles of jobs before j (work[njobs] = total),
+// first[j] = first affected tile of job j.
+struct ClassDev {
+    const Job* jobs;
+    int* work;
+    int* first;
+    int njobs;
+    int ncell;                // cells per workgroup of the class
+};
+
+// per (unit, scale): where the selection finds the scale's tile bests
+struct UnitScale {
+    long long tile_off;
     int tiles;
+    int ncell;
 };
 
 }  // namespace empi

@@ -65,6 +65,21 @@ typedef struct {
     int reserved;
 } empi_atom;
 
+/* One scale of the optimal Gabor dictionary (src/gabor.cpp:228-249). */
+typedef struct {
+    double scale;    /* s [seconds] */
+    int Nfft;
+    int tCount;
+    int fCount;
+    int reserved;
+} empi_scale_info;
+
+/* Host-only (no device call): the dictionary GaborWorkspaceBuilder would
+ * build for fs and samples.  Writes up to max_scales entries, the scale
+ * count to *count and sum F*T (Workspace::getAtomCount) to *entries. */
+empi_status empi_dict_info(const empi_dict_params* dict, double fs, int samples, empi_scale_info* scales,
+                           int max_scales, int* count, long long* entries);
+
 /* ------------------------------------------------------------ workspace */
 
 typedef struct empi_ws empi_ws;
