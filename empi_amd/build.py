@@ -25,6 +25,8 @@ LIB = os.path.join(HERE, "libempi_hip.so")
 HOST_SOURCES = ["host/empi_main.cpp", "host/workspace.cpp", "host/config.cpp", "host/book.cpp",
                 "host/signal_reader.cpp"]
 CLI = os.path.join(HERE, "empi-hip")
+TOOL_SOURCES = ["host/host_tool.cpp", "host/config.cpp", "host/book.cpp"]
+TOOL = os.path.join(HERE, "empi-host-tool")
 
 
 def _run(cmd):
@@ -52,9 +54,18 @@ def build_host():
     return CLI
 
 
+def build_tool():
+    tmp = TOOL + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+          "-I" + os.path.join(HERE, "host"), "-o", tmp] + [os.path.join(HERE, s) for s in TOOL_SOURCES])
+    os.replace(tmp, TOOL)
+    return TOOL
+
+
 def build_all():
     build_engine()
     build_host()
+    build_tool()
 
 
 if __name__ == "__main__":
