@@ -27,13 +27,13 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--channels", type=int, default=64)
     ap.add_argument("--iterations", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iterations", type=int, default=8,
+    ap.add_argument("--cpu-iterations", type=int, default=20,
                     help="iterations of the CPU-baseline sample (one channel-epoch)")
     return ap.parse_args()
 

@@ -16,6 +16,8 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -98,16 +100,37 @@ __device__ inline void dft(float2* v) {
 
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 
+// e^{2 pi i rev} from the hardware sine/cosine (argument in revolutions;
+// absolute error 1.3e-7 for |rev| <= 1.5, measured on gfx950)
+__device__ inline float2 cis_rev(float rev) {
+    return make_float2(__builtin_amdgcn_cosf(rev), __builtin_amdgcn_sinf(rev));
+}
+
+// Occupancy hint for the column kernels (waves per SIMD the register
+// allocator must allow); a build-time tuning knob.
+#ifndef EMPI_COLUMN_WAVES
+#define EMPI_COLUMN_WAVES 4   // measured best on gfx950 (k_columns, workload 2)
+#endif
+
 // Workgroup geometry of class Q: NSL slots of Q complex points in LDS,
 // 16 points per thread.
 template <int Q>
 struct Geo {
     static constexpr int NSL = Q >= 2048 ? 2 : (4096 / Q > 512 ? 512 : 4096 / Q);
     static constexpr int NPTS = NSL * Q;
-    static constexpr int NT = NPTS / 16;
+    static constexpr int EPT = Q >= 8192 ? 32 : 16;
+    static constexpr int NT = NPTS / EPT;
     static constexpr int NCELL = NSL / 2;
-    static constexpr int EPT = 16;
+    // waves per SIMD the register allocator must allow (single-channel mode;
+    // LDS caps Q = 8192 at one workgroup, 8 waves, per CU anyway)
+    static constexpr int WAVES = Q >= 8192 ? 2 : EMPI_COLUMN_WAVES;
 };
+
+// column kernel modes: one channel; channel sum of products (MMP3 and
+// multichannel workspaces); MMP1 constraint (three accumulators per bin)
+enum { kModeSingle = 0, kModeSum = 1, kModeMmp1 = 2 };
+template <int Q, int MODE>
+constexpr int column_waves() { return MODE == kModeSingle ? Geo<Q>::WAVES : MODE == kModeSum ? 2 : 1; }
 
 // LDS index with one float2 of padding per 8: the stride-8 writes of the
 // first radix-8 passes then spread over all banks.
@@ -140,9 +163,21 @@ __device__ inline void stockham_pass(float2* buf, const float2* __restrict__ tw,
         const int slot = bi / BPS, j = bi % BPS;
         const int k = j & (Ns - 1);
         if constexpr (Ns > 1) {
-            const int sh = lgTWN - ilog2c(Ns * R);
+            // w^r, w = e^{-2 pi i k / (Ns R)}, by products of the hardware w (depth <= 3)
+            float2 w[R];
+            w[1] = cis_rev(-static_cast<float>(k) * (1.0f / static_cast<float>(Ns * R)));
+            if constexpr (R >= 4) {
+                w[2] = cmul(w[1], w[1]);
+                w[3] = cmul(w[2], w[1]);
+            }
+            if constexpr (R == 8) {
+                w[4] = cmul(w[2], w[2]);
+                w[5] = cmul(w[4], w[1]);
+                w[6] = cmul(w[3], w[3]);
+                w[7] = cmul(w[4], w[3]);
+            }
 #pragma unroll
-            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[(r * k) << sh]);
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r]);
         }
         dft<R>(v[b]);
         const int base = slot * Q + (j - k) * R + k;
@@ -191,33 +226,12 @@ struct CellInfo {
     int ka, kb;     // sub-transforms of the cell's two slots, -1 = none
 };
 
-// E = exp(-x) and 1 - E without cancellation, x = 2 pi (s f4)^2
-__device__ inline void gauss_factors(float x, float* E, float* omE) {
-    *E = __expf(-x);
-    // (1 - e^-x)/x by Taylor below 0.5 (truncation < 1e-8 relative)
-    float p = -1.0f / 40320.0f;
-    p = fmaf(p, x, 1.0f / 5040.0f);
-    p = fmaf(p, x, -1.0f / 720.0f);
-    p = fmaf(p, x, 1.0f / 120.0f);
-    p = fmaf(p, x, -1.0f / 24.0f);
-    p = fmaf(p, x, 1.0f / 6.0f);
-    p = fmaf(p, x, -0.5f);
-    p = fmaf(p, x, 1.0f);
-    *omE = x < 0.5f ? x * p : 1.0f - *E;
+// e^{2 pi i cyc} from the hardware sine/cosine, whose argument is in
+// revolutions (absolute error 1.3e-7 for |cyc| <= 1.5, measured on gfx950)
+__device__ inline void twiddle_rev(float cyc, float* sn, float* cs) {
+    *sn = __builtin_amdgcn_sinf(cyc);
+    *cs = __builtin_amdgcn_cosf(cyc);
 }
-
-// product of one channel at one bin from the phase-corrected value u,
-// |u|^4 / ((1+E) Re(u)^2 + (1-E) Im(u)^2) (the constant K' applied later)
-__device__ inline float bin_term(float ur, float ui, float ja) {
-    float E, omE;
-    gauss_factors(6.283185307179586f * ja * ja, &E, &omE);
-    const float a2 = ur * ur, b2 = ui * ui;
-    const float num = a2 + b2, den = (1.0f + E) * a2 + omE * b2;
-    return den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
-}
-
-// e^{2 pi i cyc}
-__device__ inline void twiddle_rev(float cyc, float* sn, float* cs) { sincospif(2.0f * cyc, sn, cs); }
 
 __device__ inline unsigned long long shfl_xor_u64(unsigned long long v, int m) {
     const unsigned lo = __shfl_xor(static_cast<unsigned>(v), m);
@@ -282,19 +296,23 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(const ClassDev* __restric
     if (tid == kPlanThreads - 1) cd.work[n] = part[tid];
 }
 
-// MULTI = false: one channel, no constraint (SMP units, MMP2 sums);
-// MULTI = true: channel sums of squares, MMP1 constraint when ud.constraint.
+
+// MODE kModeSingle: one channel, no constraint (SMP units, MMP2 sums);
+// kModeSum: sum over channels of the products (MMP3); kModeMmp1: MMP1
+// constraint on low-frequency bins, channel sum on high ones.
 // Persistent: workgroups stride through the class's affected tiles.
-template <int Q, bool MULTI>
-__global__ __launch_bounds__(Geo<Q>::NT) void k_columns(
+template <int Q, int MODE>
+__global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(column_waves<Q, MODE>()))) void k_columns(
     const ClassDev* __restrict__ classes, int cls, const ScaleDev* __restrict__ scales,
     const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
-    const double* __restrict__ residual, unsigned long long* __restrict__ cellbest,
+    const float* __restrict__ residual, unsigned long long* __restrict__ cellbest,
     unsigned long long* __restrict__ tilebest, const float2* __restrict__ tw, int lgTWN, int full,
     unsigned long long* __restrict__ counter) {
     using G_ = Geo<Q>;
     constexpr int NT = G_::NT, NPTS = G_::NPTS, NCELL = G_::NCELL, EPT = G_::EPT;
+    constexpr bool MULTI = MODE != kModeSingle;
     constexpr int NACC = MULTI ? EPT : 1;
+    constexpr int NACC3 = MODE == kModeMmp1 ? EPT : 1;
     constexpr int SEG = 2 * Q < 64 ? 2 * Q : 64;          // lanes sharing a cell
     constexpr int GRP = 2 * Q >= NT ? 2 * Q / NT : 1;      // consecutive elements sharing a cell
     static_assert(EPT % GRP == 0, "element groups");
@@ -361,70 +379,79 @@ __global__ __launch_bounds__(Geo<Q>::NT) void k_columns(
         for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
         if (!__syncthreads_or(any)) continue;
 
-        const int shM = lgTWN - (sd.lgNfft - 1);   // tw index shift for w_M
-        const int shN = lgTWN - sd.lgNfft;         // for w_Nfft
         const float invN = 1.0f / static_cast<float>(Nfft);
-        const float xscale = st.xscale;
-        const bool mmp1 = MULTI && ud.constraint != 0;
+        const float invM = 2.0f * invN;
+        constexpr bool mmp1 = MODE == kModeMmp1;
         const int C = MULTI ? ud.C : 1;
 
-        float acc0[NACC], acc1[NACC], acc2[NACC];
+        float acc0[NACC], acc1[NACC3], acc2[NACC3];
         float accNyq = 0.0f;
 #pragma unroll
-        for (int i = 0; i < NACC; ++i) acc0[i] = acc1[i] = acc2[i] = 0.0f;
+        for (int i = 0; i < NACC; ++i) acc0[i] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NACC3; ++i) acc1[i] = acc2[i] = 0.0f;
         unsigned mycnt = 0;
 
         for (int ch = 0; ch < C; ++ch) {
-            const double* x = residual + ud.res_off + static_cast<long long>(ch) * ud.N;
+            const float* x = residual + ud.res_off + static_cast<long long>(ch) * ud.N;
             // load: packed pairs z[m] = w[2m] + i w[2m+1], pre-twiddled per slot
+            constexpr int GL = Q >= NT ? Q / NT : 1;   // consecutive pairs sharing a cell
 #pragma unroll
-            for (int i = 0; i < EPT / 2; ++i) {
-                const int p = tid + i * NT;
-                const int c = p / Q, m = p % Q;
+            for (int gi = 0; gi < EPT / 2 / GL; ++gi) {
+                const int c = (tid + gi * GL * NT) / Q;
                 const CellInfo ci = info[c];
-                float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
-                if (ci.valid) {
-                    const int n0 = 2 * m, n1 = 2 * m + 1;
-                    float w0 = 0.f, w1 = 0.f;
-                    if (n0 < ci.W) {
-                        const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
-                        w0 = static_cast<float>(x[ci.iL + n0] * xscale) * __expf(-sd.gA * u * u);
+#pragma unroll 2
+                for (int q = 0; q < GL; ++q) {
+                    const int m = (tid + (gi * GL + q) * NT) % Q;
+                    float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
+                    if (ci.valid) {
+                        const int n0 = 2 * m, n1 = 2 * m + 1;
+                        float w0 = 0.f, w1 = 0.f;
+                        if (n0 < ci.W) {
+                            const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
+                            w0 = x[ci.iL + n0] * __expf(-sd.gA * u * u);
+                        }
+                        if (n1 < ci.W) {
+                            const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
+                            w1 = x[ci.iL + n1] * __expf(-sd.gA * u * u);
+                        }
+                        const float2 z = make_float2(w0, w1);
+                        // pre-twiddle w_M^{k2 m}, k2 m < M: exact revolution counts
+                        za = ci.ka == 0 ? z : cmul(z, cis_rev(-static_cast<float>(ci.ka * m) * invM));
+                        if (ci.kb >= 0) zb = cmul(z, cis_rev(-static_cast<float>(ci.kb * m) * invM));
                     }
-                    if (n1 < ci.W) {
-                        const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
-                        w1 = static_cast<float>(x[ci.iL + n1] * xscale) * __expf(-sd.gA * u * u);
-                    }
-                    const float2 z = make_float2(w0, w1);
-                    za = ci.ka == 0 ? z : cmul(z, tw[(ci.ka * m) << shM]);
-                    if (ci.kb >= 0) zb = cmul(z, tw[(ci.kb * m) << shM]);
+                    buf[lpad((2 * c) * Q + m)] = za;
+                    buf[lpad((2 * c + 1) * Q + m)] = zb;
                 }
-                buf[lpad((2 * c) * Q + m)] = za;
-                buf[lpad((2 * c + 1) * Q + m)] = zb;
             }
             __syncthreads();
             fft_passes<Q, NT, NPTS, 1>(buf, tw, lgTWN);
 
-            // post: split the packed spectrum, phase-correct, product per bin
-            unsigned long long run = 0ull;
-#pragma unroll
-            for (int i = 0; i < EPT; ++i) {
-                const int e = tid + i * NT;
-                const int c = e / (2 * Q), rem = e % (2 * Q);
-                const int half = rem / Q, k1 = rem % Q;
-                const CellInfo& ci = info[c];
-                const int k2 = half ? ci.kb : ci.ka;
-                const int k = P * k1 + k2;
-                if (ci.valid && k2 >= 0 && k < F) {
+            // post: split the packed spectrum, phase-correct, product per bin.
+            // Elements of one group share a cell: its info is read once.
+#pragma unroll MULTI ? EPT / GRP : 1
+            for (int gi = 0; gi < EPT / GRP; ++gi) {
+                const int c = (tid + gi * GRP * NT) / (2 * Q);
+                const CellInfo ci = info[c];
+                float runp = 0.0f;
+                int runk = 0x7FFFFFFF;
+                auto element = [&](const int i) {
+                    const int e = tid + i * NT;
+                    const int rem = e % (2 * Q);
+                    const int half = rem / Q, k1 = rem % Q;
+                    const int k2 = half ? ci.kb : ci.ka;
+                    const int k = P * k1 + k2;
+                    if (!(ci.valid && k2 >= 0 && k < F)) return;
                     const int slot = 2 * c + half;
                     const float2 Zk = buf[lpad(slot * Q + k1)];
                     float2 Zp;
                     if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
                     else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
-                    const float2 A = make_float2(Zk.x + Zp.x, Zk.y - Zp.y);
-                    const float2 B = make_float2(Zk.x - Zp.x, Zk.y + Zp.y);
-                    const float2 wv = tw[k << shN];
-                    const float pr = wv.x * B.x - wv.y * B.y, pi = wv.x * B.y + wv.y * B.x;
-                    const float Xr = 0.5f * (A.x + pi), Xi = 0.5f * (A.y - pr);
+                    const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
+                    const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
+                    const float2 wv = cis_rev(-static_cast<float>(k) * invN);
+                    const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
+                    const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
                     const bool high = 4 * k > Nfft;
                     const int j = high ? M - k : k;
                     const unsigned ci_int =
@@ -433,35 +460,57 @@ __global__ __launch_bounds__(Geo<Q>::NT) void k_columns(
                     if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
                     float sn, cs;
                     twiddle_rev(cyc, &sn, &cs);
-                    const float ur = Xr * cs - Xi * sn, ui = Xr * sn + Xi * cs;
+                    // u = X e^{2 pi i cyc}; the product needs |u|^2 = |X|^2 and a = Re u
+                    const float a = Xr * cs - Xi * sn;
+                    const float num = Xr * Xr + Xi * Xi;
+                    float T1 = sd.invKp, T2 = 0.0f;
+                    if (j < sd.JT) {
+                        const float2 tt = sd.etab[j];
+                        T1 = tt.x;
+                        T2 = tt.y;
+                    }
                     if constexpr (!MULTI) {
                         ++mycnt;
-                        run = max(run, pack_best(sd.Kp * bin_term(ur, ui, static_cast<float>(j) * sd.alpha),
-                                                 static_cast<unsigned>(k)));
+                        const float den = fmaf(T1, num, T2 * a * a);
+                        const float pv = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
+                        if (pv > runp || (pv == runp && k < runk)) {
+                            runp = pv;
+                            runk = k;
+                        }
                     } else if (mmp1 && !high) {
-                        acc0[i] += ur * ur;
-                        acc1[i] += ui * ui;
-                        acc2[i] += ur * ui;
+                        if constexpr (mmp1) {
+                            const float b = Xr * sn + Xi * cs;
+                            acc0[i] += a * a;
+                            acc1[i] += b * b;
+                            acc2[i] += a * b;
+                        }
                     } else {
-                        acc0[i] += bin_term(ur, ui, static_cast<float>(j) * sd.alpha);
+                        const float den = fmaf(T1, num, T2 * a * a);
+                        acc0[i] += den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
                     }
-                }
-                if constexpr (!MULTI) {
-                    if (i % GRP == GRP - 1) {
+                };
+                if constexpr (MULTI) {
 #pragma unroll
-                        for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
-                        if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
-                        run = 0ull;
+                    for (int q = 0; q < GRP; ++q) {
+                        element(gi * GRP + q);
+                        __builtin_amdgcn_sched_barrier(0);   // keep per-element live ranges short
                     }
+                } else {
+#pragma unroll 2
+                    for (int q = 0; q < GRP; ++q) element(gi * GRP + q);
+                    unsigned long long run = runk == 0x7FFFFFFF ? 0ull : pack_best(runp, static_cast<unsigned>(runk));
+#pragma unroll
+                    for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
+                    if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
                 }
             }
-            // Nyquist bin k = M of g = 0 cells: X[M] = Re Z0 - Im Z0, u real
+            // Nyquist bin k = M of g = 0 cells: X[M] = Re Z0 - Im Z0, u real, E = 1
             if (tid < NCELL && F > M) {
                 const CellInfo& ci = info[tid];
                 if (ci.valid && ci.ka == 0) {
                     const float2 Z0 = buf[lpad((2 * tid) * Q)];
                     const float a = Z0.x - Z0.y;
-                    accNyq += 0.5f * a * a;   // |u|^4/den with E = 1: a^4 / (2 a^2)
+                    accNyq += a * a;
                 }
             }
             __syncthreads();
@@ -469,49 +518,63 @@ __global__ __launch_bounds__(Geo<Q>::NT) void k_columns(
 
         // final products of multi-channel units and per-cell maximum
         if constexpr (MULTI) {
-            unsigned long long run = 0ull;
 #pragma unroll
-            for (int i = 0; i < EPT; ++i) {
-                const int e = tid + i * NT;
-                const int c = e / (2 * Q), rem = e % (2 * Q);
-                const int half = rem / Q, k1 = rem % Q;
-                const CellInfo& ci = info[c];
-                const int k2 = half ? ci.kb : ci.ka;
-                const int k = P * k1 + k2;
-                if (ci.valid && k2 >= 0 && k < F) {
+            for (int gi = 0; gi < EPT / GRP; ++gi) {
+                const int c = (tid + gi * GRP * NT) / (2 * Q);
+                const CellInfo ci = info[c];
+                float runp = 0.0f;
+                int runk = 0x7FFFFFFF;
+#pragma unroll
+                for (int q = 0; q < GRP; ++q) {
+                    const int i = gi * GRP + q;
+                    const int e = tid + i * NT;
+                    const int rem = e % (2 * Q);
+                    const int half = rem / Q, k1 = rem % Q;
+                    const int k2 = half ? ci.kb : ci.ka;
+                    const int k = P * k1 + k2;
+                    if (!(ci.valid && k2 >= 0 && k < F)) continue;
                     ++mycnt;
                     const bool high = 4 * k > Nfft;
-                    float p;
+                    float pv;
                     if (mmp1 && !high) {
-                        const float Srr = acc0[i], Sii = acc1[i], Sri = acc2[i];
+                        float Srr = acc0[i], Sii = 0.0f, Sri = 0.0f;
+                        if constexpr (mmp1) {
+                            Sii = acc1[i];
+                            Sri = acc2[i];
+                        }
                         const float Dr = Srr - Sii, Di = 2.0f * Sri;
                         const float Dm = hypotf(Dr, Di);
                         const float lam = 0.5f * (Srr + Sii + Dm);
                         float cos2 = 1.0f;
                         if (Dm > 0.0f) cos2 = Dr >= 0.0f ? (Dm + Dr) / (2.0f * Dm) : (Di * Di) / (2.0f * Dm * (Dm - Dr));
-                        const float ja = static_cast<float>(k) * sd.alpha;
-                        float E, omE;
-                        gauss_factors(6.283185307179586f * ja * ja, &E, &omE);
-                        const float den = omE + 2.0f * E * cos2;
-                        p = den > 0.0f ? sd.Kp * lam / den : 0.0f;
+                        float T1 = sd.invKp, T2 = 0.0f;
+                        if (k < sd.JT) {
+                            const float2 tt = sd.etab[k];
+                            T1 = tt.x;
+                            T2 = tt.y;
+                        }
+                        const float den = fmaf(T2, cos2, T1);
+                        pv = den > 0.0f ? lam / den : 0.0f;
                     } else {
-                        p = sd.Kp * acc0[i];
+                        pv = acc0[i];
                     }
-                    run = max(run, pack_best(p, static_cast<unsigned>(k)));
+                    if (pv > runp || (pv == runp && k < runk)) {
+                        runp = pv;
+                        runk = k;
+                    }
                 }
-                if (i % GRP == GRP - 1) {
+                unsigned long long run = runk == 0x7FFFFFFF ? 0ull : pack_best(runp, static_cast<unsigned>(runk));
 #pragma unroll
-                    for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
-                    if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
-                    run = 0ull;
-                }
+                for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
+                if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
             }
         }
         if (tid < NCELL && F > M) {
             const CellInfo& ci = info[tid];
             if (ci.valid && ci.ka == 0) {
                 ++mycnt;
-                atomicMax(&best[tid], pack_best(sd.Kp * accNyq, static_cast<unsigned>(M)));
+                // E = 1 at Nyquist: |u|^4 / (2 E a^2 / K') = K' a^2 / 2 summed over channels
+                atomicMax(&best[tid], pack_best(0.5f * sd.Kp * accNyq, static_cast<unsigned>(M)));
             }
         }
         if (mycnt) atomicAdd(&cnt_lds, mycnt);
@@ -577,6 +640,18 @@ __device__ double unit_energy(const double* x, int C, int N, double fs, double* 
         total += block_sum<NT>(part, red) / fs;
     }
     return total;
+}
+
+// FP32 mirror of a unit's residual, scaled by the unit's power of two
+__global__ __launch_bounds__(kSelectThreads) void k_mirror(const UnitDev* __restrict__ units,
+                                                            const UnitState* __restrict__ state,
+                                                            const double* __restrict__ residual,
+                                                            float* __restrict__ res32) {
+    const UnitDev ud = units[blockIdx.x];
+    const float xs = state[blockIdx.x].xscale;
+    const long long n = static_cast<long long>(ud.C) * ud.N;
+    for (long long i = threadIdx.x; i < n; i += kSelectThreads)
+        res32[ud.res_off + i] = static_cast<float>(residual[ud.res_off + i] * xs);
 }
 
 // residual <- input (FP64), energy, input scaling; one workgroup per unit
@@ -717,7 +792,7 @@ __device__ void atom_params(const UnitDev& ud, const ScaleDev& sd, int k, int f,
 
 // x -= atom waveform (gabor.cpp:143-187 with fit = false); all threads
 template <int NT>
-__device__ void subtract_waveform(const empi_atom& a, double fs, double* x, int N) {
+__device__ void subtract_waveform(const empi_atom& a, double fs, double* x, int N, float* x32, float xscale) {
     const double sA = a.params[3] / fs;
     const double fA = a.params[4] * fs / 2.0;
     const double tA = a.params[2] / fs;
@@ -736,7 +811,9 @@ __device__ void subtract_waveform(const empi_atom& a, double fs, double* x, int 
         double w = amplitude * cos(omega * t + phase);
         const double xw = ga * i - gb;
         w *= exp(-xw * xw);
-        x[iL + i] -= w;
+        const double y = x[iL + i] - w;
+        x[iL + i] = y;
+        x32[iL + i] = static_cast<float>(y * xscale);
     }
 }
 
@@ -754,7 +831,8 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
                                                const ScaleDev* __restrict__ scales,
                                                const UnitScale* __restrict__ uscales,
                                                const unsigned long long* __restrict__ tilebest,
-                                               double* __restrict__ residual, int mode) {
+                                               double* __restrict__ residual, float* __restrict__ res32,
+                                               int mode) {
     __shared__ double red[NT];
     __shared__ float redp[NT];
     __shared__ unsigned long long redk[NT];
@@ -838,7 +916,8 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
     for (int c = 0; c < ud.C; ++c) {
         if (tid == 0) chosen = ud.atoms[static_cast<long long>(c) * ud.atom_cstride + it];
         __syncthreads();
-        subtract_waveform<NT>(chosen, ud.fs, x + static_cast<long long>(c) * ud.N, ud.N);
+        subtract_waveform<NT>(chosen, ud.fs, x + static_cast<long long>(c) * ud.N, ud.N,
+                              res32 + ud.res_off + static_cast<long long>(c) * ud.N, st.xscale);
         __syncthreads();
     }
     const double E = unit_energy<NT>(x, ud.C, ud.N, ud.fs, red);
@@ -916,12 +995,17 @@ __global__ __launch_bounds__(NT) void k_refit(const empi_atom* __restrict__ sum_
 template <int NT>
 __global__ __launch_bounds__(NT) void k_subtract_one(const UnitDev* __restrict__ units, UnitState* __restrict__ state,
                                                      const empi_atom* __restrict__ atom, int c,
-                                                     double* __restrict__ residual) {
+                                                     double* __restrict__ residual, float* __restrict__ res32) {
     const UnitDev ud = units[0];
     __shared__ empi_atom a;
     if (threadIdx.x == 0) a = *atom;
     __syncthreads();
-    subtract_waveform<NT>(a, ud.fs, residual + ud.res_off + static_cast<long long>(c) * ud.N, ud.N);
+    const long long off = ud.res_off + static_cast<long long>(c) * ud.N;
+    const float xs = state[0].xscale;
+    subtract_waveform<NT>(a, ud.fs, residual + off, ud.N, res32 + off, xs);
+    __syncthreads();
+    // the host handed in the whole channel: refresh its FP32 mirror
+    for (int n = threadIdx.x; n < ud.N; n += NT) res32[off + n] = static_cast<float>(residual[off + n] * xs);
     if (threadIdx.x == 0) {
         state[0].tA = a.params[2] / ud.fs;
         state[0].sA = a.params[3] / ud.fs;
@@ -981,36 +1065,31 @@ struct ItemInfo {
 constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
 
 typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
-                               const UnitDev* units, const UnitState* state, const double* residual,
+                               const UnitDev* units, const UnitState* state, const float* residual,
                                unsigned long long* cellbest, unsigned long long* tilebest, const float2* tw,
                                int lgTWN, int full, unsigned long long* counter);
 
-template <int Q, bool MULTI>
+template <int Q, int MODE>
 void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
-                    const UnitDev* units, const UnitState* state, const double* residual,
+                    const UnitDev* units, const UnitState* state, const float* residual,
                     unsigned long long* cellbest, unsigned long long* tilebest, const float2* tw, int lgTWN,
                     int full, unsigned long long* counter) {
-    hipLaunchKernelGGL((k_columns<Q, MULTI>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
+    hipLaunchKernelGGL((k_columns<Q, MODE>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
                        state, residual, cellbest, tilebest, tw, lgTWN, full, counter);
 }
 
 // persistent grid: enough workgroups to fill 256 CUs a few times over
 int column_grid(int Q, long long tiles) {
-    const int nt = Q >= 2048 ? (Q * 2) / 16 : 256;
+    const int nt = Q >= 8192 ? (Q * 2) / 32 : Q >= 2048 ? (Q * 2) / 16 : 256;
     const long long cap = nt >= 1024 ? 512 : nt >= 512 ? 1024 : 2048;
     return static_cast<int>(std::max(1ll, std::min(tiles, cap)));
 }
 
-// [multi][class]
-const ColumnLauncher kLaunchers[2][kNumClasses] = {
-    {launch_columns<2, false>, launch_columns<4, false>, launch_columns<8, false>, launch_columns<16, false>,
-     launch_columns<32, false>, launch_columns<64, false>, launch_columns<128, false>, launch_columns<256, false>,
-     launch_columns<512, false>, launch_columns<1024, false>, launch_columns<2048, false>,
-     launch_columns<4096, false>, launch_columns<8192, false>},
-    {launch_columns<2, true>, launch_columns<4, true>, launch_columns<8, true>, launch_columns<16, true>,
-     launch_columns<32, true>, launch_columns<64, true>, launch_columns<128, true>, launch_columns<256, true>,
-     launch_columns<512, true>, launch_columns<1024, true>, launch_columns<2048, true>, launch_columns<4096, true>,
-     launch_columns<8192, true>}};
+// [mode][class]
+const ColumnLauncher kLaunchers[3][kNumClasses] = {
+    {launch_columns<2, 0>, launch_columns<4, 0>, launch_columns<8, 0>, launch_columns<16, 0>, launch_columns<32, 0>, launch_columns<64, 0>, launch_columns<128, 0>, launch_columns<256, 0>, launch_columns<512, 0>, launch_columns<1024, 0>, launch_columns<2048, 0>, launch_columns<4096, 0>, launch_columns<8192, 0>},
+    {launch_columns<2, 1>, launch_columns<4, 1>, launch_columns<8, 1>, launch_columns<16, 1>, launch_columns<32, 1>, launch_columns<64, 1>, launch_columns<128, 1>, launch_columns<256, 1>, launch_columns<512, 1>, launch_columns<1024, 1>, launch_columns<2048, 1>, launch_columns<4096, 1>, launch_columns<8192, 1>},
+    {launch_columns<2, 2>, launch_columns<4, 2>, launch_columns<8, 2>, launch_columns<16, 2>, launch_columns<32, 2>, launch_columns<64, 2>, launch_columns<128, 2>, launch_columns<256, 2>, launch_columns<512, 2>, launch_columns<1024, 2>, launch_columns<2048, 2>, launch_columns<4096, 2>, launch_columns<8192, 2>}};
 
 int cells_per_wg(int Q) {
     const int nsl = Q >= 2048 ? 2 : std::min(512, 4096 / Q);
@@ -1025,7 +1104,7 @@ struct empi_plan {
     std::vector<int> dict_scale0;
     std::vector<UnitDev> units;
     std::vector<ScaleDev> scales_h;
-    ClassJobs classes[2][kNumClasses];   // [multi][class]
+    ClassJobs classes[3][kNumClasses];   // [mode][class]
     int maxIter = 0;
     int twN = 2, lgTWN = 1;
     long long total_in = 0;
@@ -1034,6 +1113,7 @@ struct empi_plan {
     DevBuf<UnitDev> d_units;
     DevBuf<UnitState> d_state;
     DevBuf<double> d_res;
+    DevBuf<float> d_res32;
     DevBuf<unsigned long long> d_cells;
     DevBuf<unsigned long long> d_tiles;
     DevBuf<UnitScale> d_uscales;
@@ -1041,6 +1121,7 @@ struct empi_plan {
     std::vector<UnitScale> uscales_h;
     int nclass_dev = 0;
     DevBuf<float2> d_tw;
+    DevBuf<float2> d_etab;
     DevBuf<empi_atom> d_atoms;
     DevBuf<double> d_energies;
     DevBuf<double2> d_fitv;
@@ -1055,6 +1136,11 @@ struct empi_plan {
     unsigned long long h_counter[2 * kNumClasses] = {};
     bool has_results = false;
 
+    // one MP iteration (select, plan, every column class) captured once
+    hipStream_t stream = nullptr;
+    hipGraphExec_t iter_exec = nullptr;
+    hipGraph_t iter_graph = nullptr;
+
     // optional per-class kernel timing (HIP events on the launch stream)
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[kNumClasses];
@@ -1062,6 +1148,9 @@ struct empi_plan {
     double class_ms[kNumClasses] = {};
     long long class_launches[kNumClasses] = {};
     ~empi_plan() {
+        if (iter_exec) (void)hipGraphExecDestroy(iter_exec);
+        if (iter_graph) (void)hipGraphDestroy(iter_graph);
+        if (stream) (void)hipStreamDestroy(stream);
         for (auto& v : ev)
             for (auto& e : v) {
                 (void)hipEventDestroy(e.first);
@@ -1190,6 +1279,7 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
 
     // scales of every dictionary
     int maxNfft = 2;
+    std::vector<float2> etab_h;
     for (size_t di = 0; di < p->dicts.size(); ++di) {
         const Dictionary& d = p->dicts[di];
         p->dict_scale0.push_back(static_cast<int>(p->scales_h.size()));
@@ -1213,6 +1303,21 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             s.lgNfft = lg2(sc.Nfft);
             s.alpha = static_cast<float>(sc.s * d.fs / sc.Nfft);
             s.Kp = static_cast<float>(2.0 * M_SQRT2 / (sc.s * d.fs));
+            {
+                // E = exp(-2 pi (j alpha)^2) is below 3e-25 from j alpha = 3 on: only
+                // the first bins on either side of the spectrum need it
+                const double alpha = sc.s * d.fs / sc.Nfft;
+                const double Kp = 2.0 * M_SQRT2 / (sc.s * d.fs);
+                const int JT = static_cast<int>(std::min<double>(sc.Nfft / 2 + 1, std::ceil(3.0 / alpha) + 1));
+                s.JT = JT;
+                s.invKp = static_cast<float>(1.0 / Kp);
+                s.etab = reinterpret_cast<const float2*>(static_cast<uintptr_t>(etab_h.size()));   // patched below
+                for (int j = 0; j < JT; ++j) {
+                    const double x = 2.0 * M_PI * (j * alpha) * (j * alpha);
+                    etab_h.push_back(make_float2(static_cast<float>(-std::expm1(-x) / Kp),
+                                                 static_cast<float>(2.0 * std::exp(-x) / Kp)));
+                }
+            }
             s.gA = static_cast<float>(M_PI / ((sc.s * d.fs) * (sc.s * d.fs)));
             s.Wmax = sc.Wmax;
             maxNfft = std::max(maxNfft, sc.Nfft);
@@ -1232,8 +1337,8 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
         for (int l = 0; l < u.L; ++l) {
             const ScaleDev& s = p->scales_h[u.scale0 + l];
             const int cls = s.lgQ - kMinLgQ;
-            const int multi = (u.C > 1 || u.constraint) ? 1 : 0;
-            ClassJobs& cj = p->classes[multi][cls];
+            const int mode = u.constraint ? kModeMmp1 : (u.C > 1 ? kModeSum : kModeSingle);
+            ClassJobs& cj = p->classes[mode][cls];
             Job j{};
             j.unit = static_cast<int>(ui);
             j.scale = u.scale0 + l;
@@ -1259,9 +1364,12 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_units.alloc(p->units.size())) != EMPI_OK) return st;
     if ((st = p->d_state.alloc(p->units.size())) != EMPI_OK) return st;
     if ((st = p->d_res.alloc(static_cast<size_t>(res_off))) != EMPI_OK) return st;
+    if ((st = p->d_res32.alloc(static_cast<size_t>(res_off))) != EMPI_OK) return st;
     if ((st = p->d_cells.alloc(static_cast<size_t>(cell_off))) != EMPI_OK) return st;
     if ((st = p->d_tiles.alloc(static_cast<size_t>(tile_off))) != EMPI_OK) return st;
     if ((st = p->d_uscales.alloc(p->uscales_h.size())) != EMPI_OK) return st;
+    if ((st = p->d_etab.alloc(etab_h.size())) != EMPI_OK) return st;
+    for (ScaleDev& sd : p->scales_h) sd.etab = p->d_etab.p + reinterpret_cast<uintptr_t>(sd.etab);
     if ((st = p->d_tw.alloc(static_cast<size_t>(p->twN))) != EMPI_OK) return st;
     if ((st = p->d_atoms.alloc(static_cast<size_t>(atom_off))) != EMPI_OK) return st;
     if ((st = p->d_energies.alloc(static_cast<size_t>(energy_off))) != EMPI_OK) return st;
@@ -1283,11 +1391,12 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_scales.upload(p->scales_h.data(), p->scales_h.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_units.upload(p->units.data(), p->units.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_tw.upload(tw.data(), tw.size(), s0)) != EMPI_OK) return st;
+    if ((st = p->d_etab.upload(etab_h.data(), etab_h.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_uscales.upload(p->uscales_h.data(), p->uscales_h.size(), s0)) != EMPI_OK) return st;
     std::vector<ClassDev> cds;
-    for (int multi = 0; multi < 2; ++multi) {
+    for (int mode = 0; mode < 3; ++mode) {
         for (int c = 0; c < kNumClasses; ++c) {
-            ClassJobs& cj = p->classes[multi][c];
+            ClassJobs& cj = p->classes[mode][c];
             if (cj.jobs.empty()) continue;
             if ((st = cj.d.alloc(cj.jobs.size())) != EMPI_OK) return st;
             if ((st = cj.d.upload(cj.jobs.data(), cj.jobs.size(), s0)) != EMPI_OK) return st;
@@ -1314,9 +1423,9 @@ empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
     if (p->nclass_dev == 0) return EMPI_OK;
     hipLaunchKernelGGL(k_plan, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p, p->d_state.p,
                        p->d_scales.p, full);
-    for (int mc = 0; mc < 2 * kNumClasses; ++mc) {
-        const int multi = mc / kNumClasses, c = mc % kNumClasses;
-        ClassJobs& cj = p->classes[multi][c];
+    for (int mc = 0; mc < 3 * kNumClasses; ++mc) {
+        const int mode = mc / kNumClasses, c = mc % kNumClasses;
+        ClassJobs& cj = p->classes[mode][c];
         if (cj.jobs.empty() || cj.tiles == 0) continue;
         std::pair<hipEvent_t, hipEvent_t>* evp = nullptr;
         if (p->profiling) {
@@ -1329,8 +1438,8 @@ empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
             evp = &p->ev[c][p->ev_used[c]++];
             EMPI_HIP(hipEventRecord(evp->first, s));
         }
-        kLaunchers[multi][c](column_grid(1 << (c + kMinLgQ), cj.tiles), s, p->d_classes.p, cj.slot, p->d_scales.p,
-                             p->d_units.p, p->d_state.p, p->d_res.p, p->d_cells.p, p->d_tiles.p, p->d_tw.p,
+        kLaunchers[mode][c](column_grid(1 << (c + kMinLgQ), cj.tiles), s, p->d_classes.p, cj.slot, p->d_scales.p,
+                             p->d_units.p, p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p, p->d_tw.p,
                              p->lgTWN, full, p->d_counter.p);
         if (evp) EMPI_HIP(hipEventRecord(evp->second, s));
     }
@@ -1353,9 +1462,13 @@ empi_status collect_profile(empi_plan* p) {
     return EMPI_OK;
 }
 
-empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream_t s) {
+empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream_t user) {
     const float* in = signal;
     empi_status st;
+    // work runs on the plan's stream (capturable); the caller's stream is drained first
+    EMPI_HIP(hipStreamSynchronize(user));
+    if (!p->stream) EMPI_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    hipStream_t s = p->stream;
     if (!on_device) {
         if (!p->d_input.p || p->d_input.n < static_cast<size_t>(p->total_in)) {
             if ((st = p->d_input.alloc(static_cast<size_t>(p->total_in))) != EMPI_OK) return st;
@@ -1373,15 +1486,35 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
         EMPI_HIP(hipStreamSynchronize(s));
     }
     hipLaunchKernelGGL(k_init, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p, in, p->d_res.p);
+    hipLaunchKernelGGL(k_mirror, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p, p->d_res.p,
+                       p->d_res32.p);
     EMPI_HIP(hipGetLastError());
     if ((st = launch_all_columns(p, s, 1)) != EMPI_OK) return st;
     std::vector<UnitState> hs(U);
-    for (int it = 0; it < p->maxIter; ++it) {
+    const bool graph = !p->profiling && std::getenv("EMPI_NO_GRAPH") == nullptr;
+    if (graph && !p->iter_exec) {
+        // capture one iteration; every kernel reads its per-iteration state from HBM
+        EMPI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
-                           p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, 3);
-        EMPI_HIP(hipGetLastError());
-        if (it + 1 < p->maxIter) {
-            if ((st = launch_all_columns(p, s, 0)) != EMPI_OK) return st;
+                           p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3);
+        empi_status cst = launch_all_columns(p, s, 0);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (cst != EMPI_OK) return cst;
+        if (e != hipSuccess) return EMPI_DEVICE;
+        p->iter_graph = g;
+        EMPI_HIP(hipGraphInstantiate(&p->iter_exec, g, nullptr, nullptr, 0));
+    }
+    for (int it = 0; it < p->maxIter; ++it) {
+        if (graph) {
+            EMPI_HIP(hipGraphLaunch(p->iter_exec, s));
+        } else {
+            hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
+                               p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3);
+            EMPI_HIP(hipGetLastError());
+            if (it + 1 < p->maxIter) {
+                if ((st = launch_all_columns(p, s, 0)) != EMPI_OK) return st;
+            }
         }
         if ((it & 7) == 7) {
             EMPI_HIP(hipMemcpyAsync(hs.data(), p->d_state.p, sizeof(UnitState) * U, hipMemcpyDeviceToHost, s));
@@ -1616,6 +1749,8 @@ empi_status empi_ws_compute(empi_ws* w, const double* signal) {
     us.xscale = std::ldexp(1.0f, -ex);
     EMPI_HIP(hipMemcpyAsync(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice, s));
     EMPI_HIP(hipMemsetAsync(p->d_counter.p, 0, sizeof(unsigned long long) * 2 * kNumClasses, s));
+    hipLaunchKernelGGL(k_mirror, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p, p->d_res.p,
+                       p->d_res32.p);
     empi_status st = launch_all_columns(p, s, 1);
     if (st != EMPI_OK) return st;
     EMPI_HIP(hipStreamSynchronize(s));
@@ -1643,7 +1778,7 @@ empi_status empi_ws_find_best(empi_ws* w, empi_atom* out) {
     us.status = EMPI_OK;
     EMPI_HIP(hipMemcpyAsync(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
-                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, 1);
+                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 1);
     EMPI_HIP(hipGetLastError());
     EMPI_HIP(hipMemcpyAsync(&us, p->d_state.p, sizeof us, hipMemcpyDeviceToHost, s));
     const int I = p->items[0].item.iterationMax;
@@ -1684,7 +1819,7 @@ empi_status empi_ws_subtract(empi_ws* w, const empi_atom* atom, int channel, dou
     EMPI_HIP(hipMemcpyAsync(dres, residual, sizeof(double) * w->N, hipMemcpyHostToDevice, s));
     EMPI_HIP(hipMemcpyAsync(w->d_one.p, atom, sizeof(empi_atom), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_subtract_one<kSelectThreads>, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p,
-                       p->d_state.p, w->d_one.p, channel, p->d_res.p);
+                       p->d_state.p, w->d_one.p, channel, p->d_res.p, p->d_res32.p);
     EMPI_HIP(hipGetLastError());
     EMPI_HIP(hipMemcpyAsync(residual, dres, sizeof(double) * w->N, hipMemcpyDeviceToHost, s));
     EMPI_HIP(hipStreamSynchronize(s));

@@ -1,0 +1,107 @@
+"""empi-hip CLI end to end (needs a GPU): config file in, MP5 book and the
+stdout protocol out, checked against the FP64 oracle and the reference's
+output conventions (src/empi.cpp:132-164, src/classes.cpp:41-48,83)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from empi_amd import synth
+from parity import compared_iterations
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "empi_amd", "empi-hip")
+TOOL = os.path.join(ROOT, "empi_amd", "empi-host-tool")
+
+
+def _write_case(tmp, data, cfg_extra, name="sig.bin"):
+    data.astype(np.float32).tofile(tmp / name)   # [t][channel] interleaved
+    cfg = tmp / "run.cfg"
+    cfg.write_text(cfg_extra + f"\nnameOfDataFile {tmp / name}\nnameOfOutputDirectory {tmp}\n")
+    return cfg
+
+
+def _read_book(path):
+    out = subprocess.run([TOOL, "readbook", str(path)], capture_output=True, text=True, check=True).stdout
+    rows = [line.split() for line in out.splitlines() if not line.startswith(("fs", "epoch"))]
+    return [(int(r[0]), np.array([float(v) for v in r[1:]])) for r in rows]
+
+
+@pytest.mark.parametrize("per_call", [False, True])
+def test_cli_smp_two_epochs(tmp_path, per_call):
+    print(f"\n[gpu case] CLI SMP 2 epochs x 2 channels, per_call={per_call}", flush=True)
+    N = 1280
+    sig = np.stack([np.concatenate([synth.single_channel(seed=10 + c), synth.single_channel(seed=20 + c)])
+                    for c in range(2)])                       # [channel][2N]
+    cfg = _write_case(tmp_path, sig.T.copy(),
+                      "energyError 0.01\nmaximalNumberOfIterations 20\nenergyPercent 99.0\nMP SMP\n"
+                      "samplingFrequency 128.0\nnumberOfChannels 2\nselectedChannels 1-2\n"
+                      f"numberOfSamplesInEpoch {N}\n")
+    args = [CLI] + (["--per-call"] if per_call else []) + [str(cfg)]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    lines = r.stdout.splitlines()
+    assert lines[0] == "START\t1\t2\t20\t99"
+    assert lines[-1] == " END"
+    assert lines.count("CHANNEL\t0") == 2 and lines.count("CHANNEL\t1") == 2
+    book = tmp_path / "sig_smp.b"
+    assert book.exists()
+    atoms = _read_book(book)
+    d = po.Dictionary(128.0, N, 0.01)
+    k = 0
+    for e in range(2):
+        for c in range(2):
+            x = sig[c, e * N:(e + 1) * N][None, :].astype(np.float64)
+            ora = po.decompose(d, x, po.SMP, 20, 99.0)
+            n = int(ora.counts[0])
+            got = [p for ch, p in atoms[k:k + n]]
+            assert all(ch == c + 1 for ch, _ in atoms[k:k + n])
+            m = compared_iterations(ora.gaps[0], n)
+            for i in range(m):
+                want = ora.params[0, i].astype(np.float32).astype(np.float64)
+                assert np.allclose(got[i][2:5], want[2:5], rtol=1e-6), (e, c, i)
+                assert abs(got[i][1] / want[1] - 1) < 1e-4
+            k += n
+            atom_lines = [ln for ln in lines if ln.startswith("ATOM\t")]
+            assert atom_lines[0].split("\t")[:3] == ["ATOM", "0", str(d.entries)]
+    assert k == len(atoms)
+
+
+def test_cli_errors(tmp_path):
+    print("\n[gpu case] CLI error codes", flush=True)
+    cfg = tmp_path / "bad.cfg"
+    cfg.write_text("energyError 2\n")
+    r = subprocess.run([CLI, str(cfg)], capture_output=True, text=True)
+    assert r.returncode == 1 and r.stdout.strip() == "ERROR\tinvalidEnergyErrorValue"
+    cfg.write_text("energyError 0.01\n")
+    r = subprocess.run([CLI, str(cfg)], capture_output=True, text=True)
+    assert r.stdout.strip() == "ERROR\tmissingConfigurationEntry:maximalNumberOfIterations"
+    zero = np.zeros((512, 1), np.float32)
+    cfg2 = _write_case(tmp_path, zero, "energyError 0.01\nmaximalNumberOfIterations 5\nenergyPercent 99\nMP smp\n"
+                                       "samplingFrequency 64\nnumberOfChannels 1\nselectedChannels 1\n", "z.bin")
+    r = subprocess.run([CLI, str(cfg2)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and r.stdout.splitlines()[-1] == "ERROR\tsignalIsEmpty"
+
+
+def test_cli_mmp_variants_name_and_run(tmp_path):
+    print("\n[gpu case] CLI MMP3 4 channels", flush=True)
+    x = synth.shared_multichannel(C=4, N=1280, fs=128.0, atoms=3, seed=9)
+    cfg = _write_case(tmp_path, x.T.copy(), "energyError 0.01\nmaximalNumberOfIterations 8\nenergyPercent 99\n"
+                                            "MP MMP3\nsamplingFrequency 128\nnumberOfChannels 4\nselectedChannels 1-4\n")
+    r = subprocess.run([CLI, str(cfg)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout
+    assert (tmp_path / "sig_mmp.b").exists()
+    assert "CHANNEL" not in r.stdout
+    atoms = _read_book(tmp_path / "sig_mmp.b")
+    ora = po.decompose(po.Dictionary(128.0, 1280, 0.01), x.astype(np.float64), po.MMP3, 8, 99.0)
+    n = int(ora.counts[0])
+    assert len(atoms) == 4 * n
+    m = compared_iterations(ora.gaps[0], n)
+    for c in range(4):
+        for i in range(m):
+            assert np.allclose(atoms[c * n + i][1][2:5], ora.params[c, i, 2:5].astype(np.float32), rtol=1e-6)
