@@ -1,12 +1,19 @@
-"""Benchmark: batched SMP decomposition on one MI355X.
+"""Benchmark: Matching Pursuit decomposition on one MI355X.
 
-Workload: 64 channels x 20 epochs x 5120 float32 samples at 256 Hz, each
-channel-epoch unit-variance pink noise + 8 random Gabor atoms; optimal Gabor
-dictionary with energyError 0.01, 100 iterations, energyPercent 99; FP32
-maps, FP64 energy bookkeeping.  A step decomposes the whole batch (full map
-build + MP loop) with the input already resident in HBM.
+Default workload (--workload smp-batch): 64 channels x 20 epochs x 5120
+float32 samples at 256 Hz, each channel-epoch unit-variance pink noise + 8
+random Gabor atoms; optimal Gabor dictionary with energyError 0.01, 100
+iterations, energyPercent 99; FP32 maps, FP64 energy bookkeeping.  A step
+decomposes the whole batch (full map build + MP loop) with the input already
+resident in HBM.  Other workloads:
+  single   1280 samples at 128 Hz, 5 atoms + noise 0.1, 50 iterations (SMP)
+  mmp3     32 channels x 10240 samples at 512 Hz, 12 shared atoms, MMP3,
+           energyError 0.005, 200 iterations
+  uneven   512 single-channel signals of 1024..16384 samples at 256 Hz, pink
+           noise + Poisson(6) atoms, 99% / at most 100 iterations
 
-Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+Prints one JSON line on stdout (progress goes to stderr).  --gpus above 1 is
+refused: this build runs on one GPU.
 """
 from __future__ import annotations
 
@@ -22,6 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "atoms/sec to 99% energy on 1 MI355X; proposal vs plain at stated energy error"
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -29,30 +41,101 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="smp-batch",
+                    choices=["smp-batch", "single", "mmp3", "uneven", "copy-calibration"])
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--channels", type=int, default=64)
-    ap.add_argument("--iterations", type=int, default=100)
+    ap.add_argument("--iterations", type=int, default=None)
+    ap.add_argument("--items", type=int, default=512, help="signals of the uneven workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=20,
-                    help="iterations of the CPU-baseline sample (one channel-epoch)")
+                    help="iterations of the CPU-baseline sample (smp-batch)")
     return ap.parse_args()
 
 
-def cpu_baseline(x0, fs, iters_total, cpu_iters):
-    """FP64 oracle (OpenMP over scales, own FFT) on one channel-epoch,
-    first cpu_iters iterations: atoms/s on the host cores."""
+def cpu_sample(x, fs, eps, variant, iters, percent, limit, what):
+    """FP64 oracle (OpenMP over scales, own FFT) on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     threads = min(16, os.cpu_count() or 1)
-    d = po.Dictionary(fs, len(x0), 0.01)
+    d = po.Dictionary(fs, x.shape[-1], eps)
     t0 = time.perf_counter()
-    r = po.decompose(d, x0[None, :].astype(np.float64), po.SMP, iters_total, 99.0,
-                     threads=threads, iter_limit=cpu_iters)
+    r = po.decompose(d, np.asarray(x, np.float64), variant, iters, percent, threads=threads, iter_limit=limit)
     dt = time.perf_counter() - t0
     atoms = int(r.counts.sum())
     return {"value": atoms / dt, "unit": "atoms/s", "cores": threads, "kind": "port",
-            "sample": f"1 of the channel-epochs (5120 samples), full map build + first {atoms} "
-                      f"iterations, {dt:.1f} s, FP64 oracle with its own radix-2 FFT"}
+            "sample": f"{what}: {atoms} atoms in {dt:.1f} s, FP64 oracle with its own radix-2 FFT",
+            "atoms": atoms, "seconds": dt}
+
+
+def workload(args):
+    """(specs, per-item signals, description, CPU-baseline sampler)"""
+    import empi_amd as E
+    from empi_amd import synth
+    if args.workload == "smp-batch":
+        fs, N, C, items = 256.0, 5120, args.channels, args.epochs
+        I = args.iterations or 100
+        sig = synth.batched_channels(items=items * C, N=N, fs=fs, atoms=8)
+        specs = [E.ItemSpec(C, N, fs, E.SMP, I, 99.0, 0.01) for _ in range(items)]
+        signals = [sig[i * C:(i + 1) * C] for i in range(items)]
+        desc = (f"SMP batched plain method: {C} channels x {items} epochs x {N} float32 samples at 256 Hz, "
+                f"pink noise + 8 Gabor atoms, energyError 0.01, {I} iterations, energyPercent 99")
+
+        def cpu():
+            return cpu_sample(signals[0][:1], fs, 0.01, 0, I, 99.0, args.cpu_iterations,
+                              f"1 of the channel-epochs (5120 samples), full map build + first "
+                              f"{args.cpu_iterations} iterations")
+        return specs, signals, desc, cpu
+    if args.workload == "single":
+        I = args.iterations or 50
+        x = synth.single_channel()
+        specs = [E.ItemSpec(1, 1280, 128.0, E.SMP, I, 99.0, 0.01)]
+        desc = (f"SMP single channel, 1280 float32 samples at 128 Hz, 5 atoms + noise 0.1, energyError 0.01, "
+                f"{I} iterations, energyPercent 99")
+
+        def cpu():
+            return cpu_sample(x[None, :], 128.0, 0.01, 0, I, 99.0, I, "the whole decomposition")
+        return specs, [x[None, :]], desc, cpu
+    if args.workload == "mmp3":
+        I = args.iterations or 200
+        x = synth.shared_multichannel()
+        specs = [E.ItemSpec(32, 10240, 512.0, E.MMP3, I, 99.0, 0.005)]
+        desc = (f"MMP3: 32 channels x 10240 float32 samples at 512 Hz, 12 shared Gabor atoms + noise 0.2, "
+                f"energyError 0.005, {I} iterations")
+
+        def cpu():
+            return cpu_sample(x, 512.0, 0.005, 3, I, 99.0, 1,
+                              "full map build + first iteration (FP64 maps of 32 channels)")
+        return specs, [x], desc, cpu
+    I = args.iterations or 100
+    xs = synth.uneven_batch(items=args.items)
+    specs = [E.ItemSpec(1, len(x), 256.0, E.SMP, I, 99.0, 0.01) for x in xs]
+    desc = (f"Uneven batch: {args.items} single-channel float32 signals of 1024..16384 samples at 256 Hz, "
+            f"pink noise + Poisson(6) atoms, energyError 0.01, 99% / at most {I} iterations")
+
+    def cpu():
+        atoms, secs, cores = 0, 0.0, 1
+        for x in xs[:4]:
+            r = cpu_sample(x[None, :], 256.0, 0.01, 0, I, 99.0, I, "")
+            atoms += r["atoms"]
+            secs += r["seconds"]
+            cores = r["cores"]
+        return {"value": atoms / secs, "unit": "atoms/s", "cores": cores, "kind": "port",
+                "sample": f"first 4 signals, whole decompositions ({atoms} atoms, {secs:.1f} s), "
+                          f"FP64 oracle with its own radix-2 FFT"}
+    return specs, [x[None, :] for x in xs], desc, cpu
+
+
+def copy_calibration():
+    """One device-to-device copy of exactly 1 GiB: the reference reading for
+    the FETCH_SIZE / WRITE_SIZE counters (expected 1048576 KiB each)."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel() * 4}))
 
 
 def main():
@@ -60,42 +143,42 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "copy-calibration":
+        copy_calibration()
+        return
     import torch
     import empi_amd as E
-    from empi_amd import synth
 
-    fs, N = 256.0, 5120
-    items = args.epochs
-    C = args.channels
-    I = args.iterations
-    sig = synth.batched_channels(items=items * C, N=N, fs=fs, atoms=8)   # [items*C][N]
-    specs = [E.ItemSpec(C, N, fs, E.SMP, I, 99.0, 0.01) for _ in range(items)]
+    specs, signals, desc, cpu = workload(args)
     plan = E.Plan(specs)
-    dev = torch.from_numpy(np.ascontiguousarray(sig, dtype=np.float32)).to("cuda:0")
+    flat = np.concatenate([np.asarray(s, np.float32).ravel() for s in signals])
+    progress(f"plan ready: {len(specs)} items, {flat.size} samples")
+    dev = torch.from_numpy(flat).to("cuda:0")
     torch.cuda.synchronize()
     ptr = dev.data_ptr()
 
     def step():
         plan.run(device_ptr=ptr)
 
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
         step()
+        progress(f"warmup step {i + 1} done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         step()
+        progress(f"timed step {i + 1} done")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
 
     atoms = 0
     explained = []
-    for i in range(items):
+    for i in range(len(specs)):
         r = plan.result(i)
         if r.status != 0:
             raise RuntimeError(f"item {i}: {E.STATUS_NAMES.get(r.status)}")
         atoms += int(r.counts.sum())
-        for c in range(C):
-            e = r.energies[c]
+        for e in r.energies:
             last = e[~np.isnan(e)][-1]
             explained.append(1.0 - last / e[0])
     value = atoms * args.steps / dt
@@ -104,33 +187,34 @@ def main():
     plan.set_profiling(True)
     step()
     plan.set_profiling(False)
+    progress("profiled step done")
     stats = plan.kernel_stats()
     dom = max(stats, key=lambda s: s["total_ms"])
-    bytes_per_entry = 8 * 1 + 4          # one FP32 complex map value per channel + one FP32 product
+    # algorithmic bytes: one FP32 complex map value per channel + one FP32 product per entry
+    bytes_per_launch = (8 * dom["entry_channels"] + 4 * dom["entries"]) / dom["launches"]
     avg_ms = dom["total_ms"] / dom["launches"]
-    bytes_per_launch = dom["entries"] * bytes_per_entry / dom["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "kernel": f"k_columns<{dom['sub_length']}>", "avg_launch_ms": avg_ms,
                 "entries_per_launch": dom["entries"] / dom["launches"]}
 
-    cpu = None
+    cpu_res = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sig[0].astype(np.float64), fs, I, args.cpu_iterations)
+        progress("cpu baseline sample")
+        cpu_res = cpu()
+        cpu_res.pop("atoms", None)
+        cpu_res.pop("seconds", None)
 
     out = {
-        "metric": "atoms/sec to 99% energy on 1 MI355X; proposal vs plain at stated energy error",
+        "metric": METRIC,
         "value": value, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32 maps / fp64 energy", "data": "synthetic",
-        "config": {"workload": f"SMP batched plain method: {C} channels x {items} epochs x {N} "
-                               f"float32 samples at 256 Hz, pink noise + 8 Gabor atoms, "
-                               f"energyError 0.01, {I} iterations, energyPercent 99",
-                   "method": "plain", "channel_epochs": items * C, "iterations_max": I,
-                   "atoms_per_step": atoms, "mean_explained_energy": float(np.mean(explained))},
+        "config": {"workload": desc, "method": "plain", "items": len(specs), "atoms_per_step": atoms,
+                   "mean_explained_energy": float(np.mean(explained))},
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_res,
         "kernels": stats,
     }
     print(json.dumps(out))
