@@ -32,6 +32,8 @@ EXPORTS = [
     "empi_plan_create", "empi_plan_run", "empi_plan_item_result", "empi_plan_item_atom_count",
     "empi_plan_entries_evaluated", "empi_plan_set_profiling", "empi_plan_kernel_classes",
     "empi_plan_kernel_stats", "empi_plan_destroy", "empi_strerror", "empi_version",
+    "empi_mlp_create", "empi_mlp_destroy", "empi_mlp_param_count", "empi_mlp_get_params",
+    "empi_mlp_set_params", "empi_mlp_forward", "empi_mlp_gradients", "empi_mlp_train",
 ]
 
 
@@ -100,6 +102,17 @@ def lib():
                                              dp, dp, dp]
         L.empi_plan_destroy.argtypes = [vp]
         L.empi_plan_destroy.restype = None
+        fp = C.POINTER(C.c_float)
+        L.empi_mlp_create.argtypes = [C.c_int, C.c_uint, C.POINTER(vp)]
+        L.empi_mlp_destroy.argtypes = [vp]
+        L.empi_mlp_destroy.restype = None
+        L.empi_mlp_param_count.argtypes = [vp]
+        L.empi_mlp_param_count.restype = C.c_size_t
+        L.empi_mlp_get_params.argtypes = [vp, fp]
+        L.empi_mlp_set_params.argtypes = [vp, fp]
+        L.empi_mlp_forward.argtypes = [vp, vp, C.c_int, vp, C.c_int]
+        L.empi_mlp_gradients.argtypes = [vp, fp, fp, C.c_int, fp, fp]
+        L.empi_mlp_train.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, fp]
         L.empi_strerror.argtypes = [C.c_int]
         L.empi_strerror.restype = C.c_char_p
         L.empi_version.restype = C.c_char_p
@@ -280,6 +293,67 @@ def dictionary(fs, samples, energy_error, scale_min=0.0, scale_max=math.inf, fre
     _check(lib().empi_dict_info(C.byref(dp), float(fs), int(samples), arr, n.value, C.byref(n), C.byref(e)),
            "empi_dict_info")
     return [(a.scale, a.Nfft, a.tCount, a.fCount) for a in arr[:n.value]], e.value
+
+
+class MLP:
+    """Region-scoring network (empi_mlp_*): input_dim -> 256 -> 256 -> 1."""
+
+    def __init__(self, input_dim, seed=0):
+        h = C.c_void_p()
+        _check(lib().empi_mlp_create(int(input_dim), int(seed), C.byref(h)), "empi_mlp_create")
+        self._h = h
+        self.input_dim = int(input_dim)
+
+    @property
+    def n_params(self):
+        return lib().empi_mlp_param_count(self._h)
+
+    def params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().empi_mlp_get_params(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        _check(lib().empi_mlp_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        _check(lib().empi_mlp_forward(self._h, X.ctypes.data_as(C.c_void_p), X.shape[0],
+                                      y.ctypes.data_as(C.c_void_p), 0), "empi_mlp_forward")
+        return y
+
+    def gradients(self, X, t):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().empi_mlp_gradients(self._h, X.ctypes.data_as(C.POINTER(C.c_float)),
+                                        t.ctypes.data_as(C.POINTER(C.c_float)), X.shape[0],
+                                        g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, X, t, epochs=1, batch=4096, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().empi_mlp_train(self._h, X.ctypes.data_as(C.c_void_p), t.ctypes.data_as(C.c_void_p),
+                                    X.shape[0], 0, int(epochs), int(batch), float(lr), int(seed),
+                                    C.byref(loss)), "empi_mlp_train")
+        return loss.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().empi_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def version():

@@ -19,7 +19,7 @@ HIP_FLAGS = [
     "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "csrc"),
 ]
 
-ENGINE_SOURCES = ["csrc/engine.hip", "csrc/dictionary.cpp"]
+ENGINE_SOURCES = ["csrc/engine.hip", "csrc/mlp.hip", "csrc/dictionary.cpp"]
 LIB = os.path.join(HERE, "libempi_hip.so")
 
 HOST_SOURCES = ["host/empi_main.cpp", "host/workspace.cpp", "host/config.cpp", "host/book.cpp",

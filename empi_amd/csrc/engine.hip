@@ -239,46 +239,19 @@ __device__ inline unsigned long long shfl_xor_u64(unsigned long long v, int m) {
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
-// Affected tiles of every job of one class (one workgroup per class):
-// columns with |t - tA| <= 3 (s + sA), widened by one column each side
-// (the column kernel applies the exact test per cell).
+// Affected tiles of every job of one class (one workgroup per class), one
+// work item per job: columns with |t - tA| <= 3 (s + sA), widened by one
+// column each side (the column kernel applies the exact test per cell), or
+// every column for the full build.
 constexpr int kPlanThreads = 1024;
-__global__ __launch_bounds__(kPlanThreads) void k_plan(const ClassDev* __restrict__ classes,
-                                                       const UnitState* __restrict__ state,
-                                                       const ScaleDev* __restrict__ scales, int full) {
-    __shared__ int part[kPlanThreads];
-    const ClassDev cd = classes[blockIdx.x];
+
+// exclusive prefix of the items' tile counts; work[count] = total
+__device__ void scan_items(const ClassDev& cd, int n, int* part) {
     const int tid = threadIdx.x;
-    const int n = cd.njobs;
     const int per = (n + kPlanThreads - 1) / kPlanThreads;
     const int j0 = min(n, tid * per), j1 = min(n, j0 + per);
     int sum = 0;
-    for (int j = j0; j < j1; ++j) {
-        const Job job = cd.jobs[j];
-        const UnitState st = state[job.unit];
-        int first = 0, cnt = 0;
-        if (st.active) {
-            if (full) {
-                cnt = job.tiles;
-            } else {
-                const ScaleDev sd = scales[job.scale];
-                const double R = (sd.s + st.sA) * kGaussHalfWidth;
-                const double scale = (sd.T - 1) / sd.tMax;
-                long lo = static_cast<long>(ceil((st.tA - R) * scale)) - 1;
-                long hi = static_cast<long>(floor((st.tA + R) * scale)) + 1;
-                lo = lo < 0 ? 0 : lo;
-                hi = hi > sd.T - 1 ? sd.T - 1 : hi;
-                if (lo <= hi) {
-                    const long c0 = lo * sd.G, c1 = hi * sd.G + sd.G - 1;
-                    first = static_cast<int>(c0 / cd.ncell);
-                    cnt = static_cast<int>(c1 / cd.ncell) - first + 1;
-                }
-            }
-        }
-        cd.first[j] = first;
-        cd.work[j] = cnt;
-        sum += cnt;
-    }
+    for (int j = j0; j < j1; ++j) sum += cd.items[j].tiles;
     part[tid] = sum;
     __syncthreads();
     for (int off = 1; off < kPlanThreads; off <<= 1) {   // inclusive scan
@@ -289,13 +262,61 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(const ClassDev* __restric
     }
     int run = part[tid] - sum;
     for (int j = j0; j < j1; ++j) {
-        const int c = cd.work[j];
         cd.work[j] = run;
-        run += c;
+        run += cd.items[j].tiles;
     }
     if (tid == kPlanThreads - 1) cd.work[n] = part[tid];
 }
 
+__global__ __launch_bounds__(kPlanThreads) void k_plan(const ClassDev* __restrict__ classes,
+                                                       const UnitState* __restrict__ state,
+                                                       const ScaleDev* __restrict__ scales, int full) {
+    __shared__ int part[kPlanThreads];
+    const ClassDev cd = classes[blockIdx.x];
+    const int n = cd.njobs;
+    for (int j = threadIdx.x; j < n; j += kPlanThreads) {
+        const Job job = cd.jobs[j];
+        const UnitState st = state[job.unit];
+        const ScaleDev sd = scales[job.scale];
+        WorkItem it{};
+        it.job = j;
+        it.atom = full ? 0 : 1;
+        it.region = -1;
+        it.tlo = 0;
+        it.thi = sd.T - 1;
+        if (st.active) {
+            if (full) {
+                it.tiles = job.tiles;
+            } else {
+                const double R = (sd.s + st.sA) * kGaussHalfWidth;
+                const double scale = (sd.T - 1) / sd.tMax;
+                long lo = static_cast<long>(ceil((st.tA - R) * scale)) - 1;
+                long hi = static_cast<long>(floor((st.tA + R) * scale)) + 1;
+                lo = lo < 0 ? 0 : lo;
+                hi = hi > sd.T - 1 ? sd.T - 1 : hi;
+                if (lo <= hi) {
+                    const long c0 = lo * sd.G, c1 = hi * sd.G + sd.G - 1;
+                    it.tile = static_cast<int>(c0 / cd.ncell);
+                    it.tiles = static_cast<int>(c1 / cd.ncell) - it.tile + 1;
+                    it.tlo = static_cast<int>(lo);
+                    it.thi = static_cast<int>(hi);
+                }
+            }
+        }
+        cd.items[j] = it;
+    }
+    if (threadIdx.x == 0) *cd.count = n;
+    __syncthreads();
+    scan_items(cd, n, part);
+}
+
+// prefix over items appended by k_region_items (one workgroup per class)
+__global__ __launch_bounds__(kPlanThreads) void k_scan_items(const ClassDev* __restrict__ classes) {
+    __shared__ int part[kPlanThreads];
+    const ClassDev cd = classes[blockIdx.x];
+    const int n = min(*cd.count, cd.max_items);
+    scan_items(cd, n, part);
+}
 
 // MODE kModeSingle: one channel, no constraint (SMP units, MMP2 sums);
 // kModeSum: sum over channels of the products (MMP3); kModeMmp1: MMP1
@@ -325,18 +346,28 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const ClassDev cd = classes[cls];
-    const int total = cd.work[cd.njobs];
+    const int nitems = min(*cd.count, cd.max_items);
+    const int total = cd.work[nitems];
     unsigned long long evaluated = 0, evaluated_ch = 0;
 
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
-        // job: last j with work[j] <= w
-        int lo = 0, hi = cd.njobs - 1;
+    // XCD-aware walk: workgroups are dealt round-robin over the 8 XCDs, so
+    // workgroup b runs on XCD b % 8; each XCD takes one contiguous eighth of
+    // the work list, keeping neighbouring cells (overlapping windows) in one L2
+    constexpr int kXcds = 8;
+    const int xcd = blockIdx.x % kXcds;
+    const int per_xcd = gridDim.x / kXcds;
+    const int chunk = (total + kXcds - 1) / kXcds;
+    const int w_end = min(total, (xcd + 1) * chunk);
+    for (int w = xcd * chunk + static_cast<int>(blockIdx.x / kXcds); w < w_end; w += per_xcd) {
+        // item: last i with work[i] <= w
+        int lo = 0, hi = nitems - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (cd.work[mid] <= w) lo = mid; else hi = mid - 1;
         }
-        const Job job = cd.jobs[lo];
-        const int tile = cd.first[lo] + (w - cd.work[lo]);
+        const WorkItem item = cd.items[lo];
+        const Job job = cd.jobs[item.job];
+        const int tile = item.tile + (w - cd.work[lo]);
         const UnitState st = state[job.unit];
         const ScaleDev sd = scales[job.scale];
         const UnitDev ud = units[job.unit];
@@ -355,7 +386,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
                 const int t = ci.cell / G;
                 const int g = ci.cell - t * G;
                 const double t0 = t * sd.tMax / (sd.T - 1);
-                if (!full) ci.valid = fabs(t0 - st.tA) <= (s + st.sA) * kGaussHalfWidth;
+                ci.valid = t >= item.tlo && t <= item.thi;
+                if (item.atom) ci.valid = ci.valid && fabs(t0 - st.tA) <= (s + st.sA) * kGaussHalfWidth;
                 const double hw = kGaussHalfWidth * s;
                 const long l = lrint((t0 - hw) * fs);
                 const long r = lrint((t0 + hw) * fs);
@@ -817,6 +849,22 @@ __device__ void subtract_waveform(const empi_atom& a, double fs, double* x, int 
     }
 }
 
+// proposal regions (see the proposal section below)
+constexpr int kBlock = 256;          // samples per region block
+constexpr int kFeatBins = 64;
+constexpr int kFeatDim = 80;         // network input
+constexpr int kRegionThreads = 256;
+
+struct RegionState {
+    float rmax;                      // exact max product at the last evaluation
+    float rbound;                    // certified bound on the current max (dirty)
+    float rscore;                    // network score (predicted log max / residual)
+    int flags;                       // bit 0 dirty, bit 1 evaluate now
+    unsigned long long rbest;        // packed (product bits, ~bin) of the best cell
+    int rcell;                       // best cell index within the scale
+    int pad;
+};
+
 struct SelBest {
     float p;
     unsigned long long key;   // (scale << 52) | (bin << 26) | t
@@ -832,7 +880,7 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
                                                const UnitScale* __restrict__ uscales,
                                                const unsigned long long* __restrict__ tilebest,
                                                double* __restrict__ residual, float* __restrict__ res32,
-                                               int mode) {
+                                               int mode, const RegionState* __restrict__ regions) {
     __shared__ double red[NT];
     __shared__ float redp[NT];
     __shared__ unsigned long long redk[NT];
@@ -844,9 +892,24 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
     const UnitDev ud = units[u];
     const int it = st.iteration;
 
-    // 1. argmax over the tile bests: product desc, then scale, bin, time asc
+    // 1. argmax over the tile bests (over the clean regions' bests with the
+    // proposal): product desc, then scale, bin, time asc
     SelBest b{-1.0f, ~0ull};
-    for (int l = 0; l < ud.L; ++l) {
+    if (regions) {
+        for (int r = tid; r < ud.L * ud.B; r += NT) {
+            const RegionState R = regions[ud.reg_off + r];
+            if ((R.flags & 1) || R.rbest == 0ull) continue;
+            const int l = r / ud.B;
+            const int G = scales[ud.scale0 + l].G;
+            SelBest c;
+            c.p = __uint_as_float(static_cast<unsigned>(R.rbest >> 32));
+            const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(R.rbest & 0xFFFFFFFFu);
+            c.key = (static_cast<unsigned long long>(l) << 52) | (static_cast<unsigned long long>(bin) << 26) |
+                    static_cast<unsigned long long>(R.rcell / G);
+            if (sel_better(c, b)) b = c;
+        }
+    }
+    for (int l = 0; l < (regions ? 0 : ud.L); ++l) {
         const UnitScale us = uscales[ud.us_off + l];
         const int G = scales[ud.scale0 + l].G;
         for (int i = tid; i < us.tiles; i += NT) {
@@ -930,6 +993,363 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
         ud.energies[it + 1] = E;
         if (E / st.totalEnergy <= ud.residualEnergy) st.active = 0;
         state[u] = st;
+    }
+}
+
+// ------------------------------------------------------------ proposal
+//
+// Regions: (scale, 256-sample block) of one unit.  A region is clean while
+// every cell in it holds its exact product; subtracting an atom makes the
+// regions it overlaps (the reference's own rule) dirty.  Per iteration: the
+// network scores dirty regions, the top-K of them plus every dirty region of
+// a global scale (windows spanning a quarter of the signal) are evaluated
+// exactly, then every dirty region whose certified bound
+// csnorm * (residual energy under its windows) exceeds P* / (1 - delta) is
+// evaluated too (the full-search fallback), so the chosen atom is within
+// (1 - delta) of the exact maximum.
+
+// affected time-index range of scale sd for the unit's last atom (widened by one)
+__device__ inline bool atom_range(const ScaleDev& sd, const UnitState& st, int* lo, int* hi) {
+    const double R = (sd.s + st.sA) * kGaussHalfWidth;
+    const double scale = (sd.T - 1) / sd.tMax;
+    long l = static_cast<long>(ceil((st.tA - R) * scale)) - 1;
+    long h = static_cast<long>(floor((st.tA + R) * scale)) + 1;
+    l = l < 0 ? 0 : l;
+    h = h > sd.T - 1 ? sd.T - 1 : h;
+    *lo = static_cast<int>(l);
+    *hi = static_cast<int>(h);
+    return l <= h;
+}
+
+// region bests from the cell bests: all regions, or those flagged "evaluate"; one wave per region
+__global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev* __restrict__ units,
+                                                                   const UnitState* __restrict__ state,
+                                                                   const ScaleDev* __restrict__ scales,
+                                                                   const RegionDev* __restrict__ regtab,
+                                                                   const unsigned long long* __restrict__ cellbest,
+                                                                   RegionState* __restrict__ regions, int all) {
+    const int u = blockIdx.x;
+    if (!all && !state[u].active) return;
+    const UnitDev ud = units[u];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nreg = ud.L * ud.B;
+    for (int r = wave; r < nreg; r += kRegionThreads / 64) {
+        RegionState* R = &regions[ud.reg_off + r];
+        if (!all && !(R->flags & 2)) continue;
+        const int l = r / ud.B;
+        const RegionDev rt = regtab[ud.regtab_off + r];
+        const ScaleDev& sd = scales[ud.scale0 + l];
+        unsigned long long best = 0ull;
+        int bcell = 0x7FFFFFFF;
+        if (rt.tlo <= rt.thi) {
+            const long long base = ud.cell_off + sd.cells_before;
+            const int c0 = rt.tlo * sd.G, c1 = rt.thi * sd.G + sd.G - 1;
+            for (int c = c0 + lane; c <= c1; c += 64) {
+                const unsigned long long v = cellbest[base + c];
+                if (v > best || (v == best && c < bcell)) {
+                    best = v;
+                    bcell = c;
+                }
+            }
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            const unsigned long long ov = shfl_xor_u64(best, off);
+            const int oc = __shfl_xor(bcell, off);
+            if (ov > best || (ov == best && oc < bcell)) {
+                best = ov;
+                bcell = oc;
+            }
+        }
+        if (lane == 0) {
+            R->rbest = best;
+            R->rmax = __uint_as_float(static_cast<unsigned>(best >> 32));
+            R->rcell = bcell;
+            R->flags = 0;
+        }
+    }
+}
+
+// regions the unit's last atom overlaps become dirty
+__global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __restrict__ units,
+                                                               const UnitState* __restrict__ state,
+                                                               const ScaleDev* __restrict__ scales,
+                                                               const RegionDev* __restrict__ regtab,
+                                                               RegionState* __restrict__ regions) {
+    const int u = blockIdx.x;
+    const UnitState st = state[u];
+    if (!st.active) return;
+    const UnitDev ud = units[u];
+    for (int r = threadIdx.x; r < ud.L * ud.B; r += kRegionThreads) {
+        const RegionDev rt = regtab[ud.regtab_off + r];
+        if (rt.tlo > rt.thi) continue;
+        const ScaleDev& sd = scales[ud.scale0 + r / ud.B];
+        int lo, hi;
+        if (atom_range(sd, st, &lo, &hi) && rt.thi >= lo && rt.tlo <= hi) regions[ud.reg_off + r].flags |= 1;
+    }
+}
+
+// Prefix energies, block spectra of the blocks the last atom touched (all
+// blocks on init), certified bounds and network rows of dirty non-global
+// regions.  Row features: 64 pooled log powers of the 256-sample block, log
+// bound, log stale max, 8 scale features, block fill, log residual/total;
+// every energy relative to the current residual energy.
+__global__ __launch_bounds__(kRegionThreads) void k_features(
+    const UnitDev* __restrict__ units, const UnitState* __restrict__ state, const ScaleDev* __restrict__ scales,
+    const RegionDev* __restrict__ regtab, const float* __restrict__ res32, RegionState* __restrict__ regions,
+    double* __restrict__ prefix, const long long* __restrict__ prefix_off, float* __restrict__ blockfeat,
+    const long long* __restrict__ feat_off, float* __restrict__ rows, int* __restrict__ rowmap,
+    float* __restrict__ rowref, int* __restrict__ rowcount, int max_rows, int init) {
+    __shared__ double part[kRegionThreads];
+    const int u = blockIdx.x;
+    const UnitState st = state[u];
+    if (!st.active) return;
+    const UnitDev ud = units[u];
+    const int N = ud.N;
+    const float* x = res32 + ud.res_off;
+    double* pre = prefix + prefix_off[u];
+    float* bf = blockfeat + feat_off[u];
+    const int tid = threadIdx.x;
+    // (a) prefix sums of x^2 (FP64), pre[n] = sum_{i<n} x_i^2
+    const int per = (N + kRegionThreads - 1) / kRegionThreads;
+    const int n0 = min(N, tid * per), n1 = min(N, n0 + per);
+    double sum = 0.0;
+    for (int n = n0; n < n1; ++n) sum += static_cast<double>(x[n]) * x[n];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < kRegionThreads; off <<= 1) {
+        const double v = tid >= off ? part[tid - off] : 0.0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    double run = part[tid] - sum;
+    for (int n = n0; n < n1; ++n) {
+        pre[n] = run;
+        run += static_cast<double>(x[n]) * x[n];
+    }
+    if (tid == kRegionThreads - 1) pre[N] = part[tid];
+    __syncthreads();
+    const double ref = pre[N] > 0.0 ? pre[N] : 1.0;
+    // (b) block spectra: 64 threads per block, bins 2m+1 and 2m+2 of a 256-point DFT
+    int b0 = 0, b1 = ud.B - 1;
+    if (!init) {
+        const double hw = kGaussHalfWidth * st.sA;
+        b0 = max(0, static_cast<int>(lrint((st.tA - hw) * ud.fs)) / kBlock);
+        b1 = min(ud.B - 1, static_cast<int>(lrint((st.tA + hw) * ud.fs)) / kBlock);
+    }
+    for (int b = b0 + (tid >> 6); b <= b1; b += kRegionThreads / 64) {
+        const int m = tid & 63;
+        float p = 0.0f;
+        for (int q = 1; q <= 2; ++q) {
+            const int k = 2 * m + q;
+            float re = 0.0f, im = 0.0f;
+            for (int n = 0; n < kBlock; ++n) {
+                const int i = b * kBlock + n;
+                const float v = i < N ? x[i] : 0.0f;
+                const float2 w = cis_rev(-static_cast<float>((k * n) & (kBlock - 1)) * (1.0f / kBlock));
+                re = fmaf(v, w.x, re);
+                im = fmaf(v, w.y, im);
+            }
+            p += re * re + im * im;
+        }
+        bf[b * kFeatBins + m] = logf(1e-30f + p / static_cast<float>(ref));
+    }
+    __syncthreads();
+    // (c) bounds and rows of dirty regions on non-global scales
+    const float eref = static_cast<float>(st.totalEnergy * ud.fs) * st.xscale * st.xscale;
+    for (int r = tid; r < ud.L * ud.B; r += kRegionThreads) {
+        RegionState* R = &regions[ud.reg_off + r];
+        if (!(R->flags & 1)) continue;
+        const int l = r / ud.B, b = r % ud.B;
+        const ScaleDev& sd = scales[ud.scale0 + l];
+        if (sd.global) continue;
+        const RegionDev rt = regtab[ud.regtab_off + r];
+        // union of the windows of columns tlo..thi
+        const double t0 = rt.tlo * sd.tMax / (sd.T - 1), t1 = rt.thi * sd.tMax / (sd.T - 1);
+        const double hw = kGaussHalfWidth * sd.s;
+        const long l0 = lrint((t0 - hw) * sd.fs), r1 = lrint((t1 + hw) * sd.fs);
+        const int iL = static_cast<int>(l0 > 0 ? l0 : 0), iR = static_cast<int>(r1 < N - 1 ? r1 : N - 1);
+        const float bound = sd.csnorm * static_cast<float>(pre[iR + 1] - pre[iL]);
+        R->rbound = bound;
+        const int row = atomicAdd(rowcount, 1);
+        if (row >= max_rows) continue;
+        float* f = rows + static_cast<long long>(row) * kFeatDim;
+        for (int q = 0; q < kFeatBins; ++q) f[q] = bf[b * kFeatBins + q];
+        f[64] = logf(1e-30f + bound / static_cast<float>(ref));
+        f[65] = logf(1e-30f + R->rmax / static_cast<float>(ref));
+        const float ls = log2f(sd.s * sd.fs) * (1.0f / 16.0f);
+        f[66] = ls;
+        f[67] = ls * ls;
+        f[68] = __builtin_amdgcn_sinf(0.5f * ls);
+        f[69] = __builtin_amdgcn_cosf(0.5f * ls);
+        f[70] = __builtin_amdgcn_sinf(ls);
+        f[71] = __builtin_amdgcn_cosf(ls);
+        f[72] = __builtin_amdgcn_sinf(2.0f * ls);
+        f[73] = __builtin_amdgcn_cosf(2.0f * ls);
+        f[74] = static_cast<float>(min(kBlock, N - b * kBlock)) / kBlock;
+        f[75] = logf(1e-30f + static_cast<float>(ref) / (eref > 0.0f ? eref : 1.0f));
+        f[76] = f[77] = f[78] = f[79] = 0.0f;
+        rowmap[row] = static_cast<int>(ud.reg_off + r);
+        rowref[row] = static_cast<float>(ref);
+    }
+}
+
+__global__ void k_scores_scatter(const float* __restrict__ y, const int* __restrict__ rowmap,
+                                 const int* __restrict__ rowcount, int max_rows, RegionState* __restrict__ regions) {
+    const int n = min(*rowcount, max_rows);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        regions[rowmap[i]].rscore = y[i];
+}
+
+// evaluate: every dirty region of a global scale, and the top-K dirty
+// regions of the others by network score (by stale max without a network)
+__global__ __launch_bounds__(kRegionThreads) void k_choose(const UnitDev* __restrict__ units,
+                                                           const UnitState* __restrict__ state,
+                                                           const ScaleDev* __restrict__ scales,
+                                                           RegionState* __restrict__ regions, int K, int by_score,
+                                                           int all) {
+    __shared__ float bv[kRegionThreads];
+    __shared__ int bi[kRegionThreads];
+    const int u = blockIdx.x;
+    if (!state[u].active) return;
+    const UnitDev ud = units[u];
+    const int tid = threadIdx.x;
+    const int nreg = ud.L * ud.B;
+    for (int r = tid; r < nreg; r += kRegionThreads) {
+        RegionState* R = &regions[ud.reg_off + r];
+        if ((R->flags & 1) && (all || scales[ud.scale0 + r / ud.B].global)) R->flags |= 2;
+    }
+    if (all) return;
+    __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        float best = -INFINITY;
+        int bidx = -1;
+        for (int r = tid; r < nreg; r += kRegionThreads) {
+            const RegionState* R = &regions[ud.reg_off + r];
+            if ((R->flags & 3) != 1) continue;
+            const float v = by_score ? R->rscore : R->rmax;
+            if (v > best || (v == best && r < bidx)) {
+                best = v;
+                bidx = r;
+            }
+        }
+        bv[tid] = best;
+        bi[tid] = bidx;
+        __syncthreads();
+        for (int off = kRegionThreads / 2; off > 0; off >>= 1) {
+            if (tid < off) {
+                const float ov = bv[tid + off];
+                const int oi = bi[tid + off];
+                if (oi >= 0 && (ov > bv[tid] || bi[tid] < 0 || (ov == bv[tid] && oi < bi[tid]))) {
+                    bv[tid] = ov;
+                    bi[tid] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0 && bi[0] >= 0) regions[ud.reg_off + bi[0]].flags |= 2;
+        __syncthreads();
+        if (bi[0] < 0) break;
+    }
+}
+
+__global__ void k_reset_counts(const ClassDev* __restrict__ classes, int n, int* __restrict__ extra) {
+    const int i = threadIdx.x;
+    if (i < n) *classes[i].count = 0;
+    if (i == 0 && extra) *extra = 0;
+}
+
+// work items of the regions flagged "evaluate" (one workgroup per unit)
+__global__ __launch_bounds__(kRegionThreads) void k_region_items(const ClassDev* __restrict__ classes,
+                                                                 const UnitDev* __restrict__ units,
+                                                                 const UnitState* __restrict__ state,
+                                                                 const ScaleDev* __restrict__ scales,
+                                                                 const UnitScale* __restrict__ uscales,
+                                                                 const RegionDev* __restrict__ regtab,
+                                                                 const RegionState* __restrict__ regions) {
+    const int u = blockIdx.x;
+    if (!state[u].active) return;
+    const UnitDev ud = units[u];
+    for (int r = threadIdx.x; r < ud.L * ud.B; r += kRegionThreads) {
+        if (!(regions[ud.reg_off + r].flags & 2)) continue;
+        const RegionDev rt = regtab[ud.regtab_off + r];
+        if (rt.tlo > rt.thi) continue;
+        const int l = r / ud.B;
+        const UnitScale us = uscales[ud.us_off + l];
+        const ClassDev& cd = classes[us.cls_slot];
+        const ScaleDev& sd = scales[ud.scale0 + l];
+        WorkItem it{};
+        it.job = us.job;
+        it.tile = rt.tlo * sd.G / cd.ncell;
+        it.tiles = (rt.thi * sd.G + sd.G - 1) / cd.ncell - it.tile + 1;
+        it.tlo = rt.tlo;
+        it.thi = rt.thi;
+        it.atom = 0;
+        it.region = static_cast<int>(ud.reg_off + r);
+        const int pos = atomicAdd(cd.count, 1);
+        if (pos < cd.max_items) cd.items[pos] = it;
+    }
+}
+
+// P* = best clean region; dirty regions whose bound exceeds P*/(1-delta) are evaluated
+__global__ __launch_bounds__(kRegionThreads) void k_certify(const UnitDev* __restrict__ units,
+                                                            const UnitState* __restrict__ state,
+                                                            RegionState* __restrict__ regions, float delta,
+                                                            unsigned long long* __restrict__ stats) {
+    __shared__ float red[kRegionThreads];
+    __shared__ int viol;
+    const int u = blockIdx.x;
+    if (!state[u].active) return;
+    const UnitDev ud = units[u];
+    const int tid = threadIdx.x;
+    const int nreg = ud.L * ud.B;
+    float best = 0.0f;
+    for (int r = tid; r < nreg; r += kRegionThreads) {
+        const RegionState* R = &regions[ud.reg_off + r];
+        if (!(R->flags & 1)) best = fmaxf(best, R->rmax);
+    }
+    red[tid] = best;
+    if (tid == 0) viol = 0;
+    __syncthreads();
+    for (int off = kRegionThreads / 2; off > 0; off >>= 1) {
+        if (tid < off) red[tid] = fmaxf(red[tid], red[tid + off]);
+        __syncthreads();
+    }
+    const float limit = red[0] / (1.0f - delta);
+    int mine = 0, dirty = 0;
+    for (int r = tid; r < nreg; r += kRegionThreads) {
+        RegionState* R = &regions[ud.reg_off + r];
+        if (!(R->flags & 1)) continue;
+        ++dirty;
+        if (R->rbound > limit) {
+            R->flags |= 2;
+            ++mine;
+        }
+    }
+    if (mine) {
+        atomicAdd(&viol, 1);
+        atomicAdd(&stats[1], static_cast<unsigned long long>(mine));
+    }
+    atomicAdd(&stats[3], static_cast<unsigned long long>(dirty));
+    __syncthreads();
+    if (tid == 0) {
+        atomicAdd(&stats[2], 1ull);
+        if (viol) atomicAdd(&stats[0], 1ull);
+    }
+}
+
+// training rows: targets log(new exact max / residual) of the rows k_features wrote
+__global__ void k_collect_rows(const float* __restrict__ rows, const int* __restrict__ rowmap,
+                               const float* __restrict__ rowref, const int* __restrict__ rowcount, int max_rows,
+                               const RegionState* __restrict__ regions, float* __restrict__ outX,
+                               float* __restrict__ outT, unsigned long long* __restrict__ outCount,
+                               unsigned long long capacity) {
+    const int n = min(*rowcount, max_rows);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned long long pos = atomicAdd(outCount, 1ull);
+        if (pos >= capacity) continue;
+        for (int q = 0; q < kFeatDim; ++q) outX[pos * kFeatDim + q] = rows[static_cast<long long>(i) * kFeatDim + q];
+        outT[pos] = logf(1e-30f + regions[rowmap[i]].rmax / rowref[i]);
     }
 }
 
@@ -1046,8 +1466,10 @@ struct ClassJobs {
     long long tiles = 0;       // all tiles of the class (full build)
     int slot = -1;             // index in the device ClassDev array
     DevBuf<Job> d;
-    DevBuf<int> work;          // [njobs + 1]
-    DevBuf<int> first;         // [njobs]
+    DevBuf<WorkItem> items;    // [max_items]
+    DevBuf<int> work;          // [max_items + 1]
+    DevBuf<int> count;         // [1]
+    int max_items = 0;         // jobs, or regions of the class in proposal mode
 };
 
 struct ItemInfo {
@@ -1079,10 +1501,12 @@ void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, c
 }
 
 // persistent grid: enough workgroups to fill 256 CUs a few times over
+// (a multiple of the 8 XCDs, which the kernel's work walk relies on)
 int column_grid(int Q, long long tiles) {
     const int nt = Q >= 8192 ? (Q * 2) / 32 : Q >= 2048 ? (Q * 2) / 16 : 256;
     const long long cap = nt >= 1024 ? 512 : nt >= 512 ? 1024 : 2048;
-    return static_cast<int>(std::max(1ll, std::min(tiles, cap)));
+    const long long g = std::max(1ll, std::min(tiles, cap));
+    return static_cast<int>((g + 7) / 8 * 8);
 }
 
 // [mode][class]
@@ -1400,14 +1824,19 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             if (cj.jobs.empty()) continue;
             if ((st = cj.d.alloc(cj.jobs.size())) != EMPI_OK) return st;
             if ((st = cj.d.upload(cj.jobs.data(), cj.jobs.size(), s0)) != EMPI_OK) return st;
-            if ((st = cj.work.alloc(cj.jobs.size() + 1)) != EMPI_OK) return st;
-            if ((st = cj.first.alloc(cj.jobs.size())) != EMPI_OK) return st;
+            cj.max_items = std::max<int>(static_cast<int>(cj.jobs.size()), cj.max_items);
+            if ((st = cj.items.alloc(static_cast<size_t>(cj.max_items))) != EMPI_OK) return st;
+            if ((st = cj.work.alloc(static_cast<size_t>(cj.max_items) + 1)) != EMPI_OK) return st;
+            if ((st = cj.count.alloc(1)) != EMPI_OK) return st;
             ClassDev cd{};
             cd.jobs = cj.d.p;
+            cd.items = cj.items.p;
             cd.work = cj.work.p;
-            cd.first = cj.first.p;
+            cd.count = cj.count.p;
             cd.njobs = static_cast<int>(cj.jobs.size());
+            cd.max_items = cj.max_items;
             cd.ncell = cells_per_wg(1 << (c + kMinLgQ));
+            cd.cls = c;
             cj.slot = static_cast<int>(cds.size());
             cds.push_back(cd);
         }
@@ -1496,7 +1925,7 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
         // capture one iteration; every kernel reads its per-iteration state from HBM
         EMPI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
-                           p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3);
+                           p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3, nullptr);
         empi_status cst = launch_all_columns(p, s, 0);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &g);
@@ -1510,7 +1939,7 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
             EMPI_HIP(hipGraphLaunch(p->iter_exec, s));
         } else {
             hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
-                               p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3);
+                               p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3, nullptr);
             EMPI_HIP(hipGetLastError());
             if (it + 1 < p->maxIter) {
                 if ((st = launch_all_columns(p, s, 0)) != EMPI_OK) return st;
@@ -1778,7 +2207,7 @@ empi_status empi_ws_find_best(empi_ws* w, empi_atom* out) {
     us.status = EMPI_OK;
     EMPI_HIP(hipMemcpyAsync(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
-                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 1);
+                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 1, nullptr);
     EMPI_HIP(hipGetLastError());
     EMPI_HIP(hipMemcpyAsync(&us, p->d_state.p, sizeof us, hipMemcpyDeviceToHost, s));
     const int I = p->items[0].item.iterationMax;

@@ -32,6 +32,8 @@ struct ScaleDev {
     float invKp;              // 1/K'
     int JT;                   // bins j < JT carry E = exp(-2 pi (j alpha)^2) > 0 in FP32
     const float2* etab;       // [JT]: ((1 - E)/K', 2E/K') in FP32 from FP64
+    int global;               // windows span a quarter of the signal or more
+    float csnorm;             // max discrete/continuous atom-norm ratio: P <= csnorm * window energy
 };
 
 struct UnitDev {
@@ -53,6 +55,9 @@ struct UnitDev {
     double* energies;         // [iterMax + 1]
     double2* fitv;            // scratch: C complex values of the chosen column
     long long us_off;         // first UnitScale of the unit
+    long long reg_off;        // first region of the unit (proposal mode)
+    int B;                    // 256-sample blocks of the unit
+    int regtab_off;           // the dictionary's region table: [scale][block]
 };
 
 struct UnitState {
@@ -73,18 +78,37 @@ struct Job {                  // one (unit, scale) pair in a kernel class
     long long cell_off;       // absolute cellbest offset of this scale
     long long tile_off;       // absolute tilebest offset of this scale
     int tiles;                // ceil(T*G / cells per workgroup)
-    int pad;
+    int lscale;               // scale index inside the unit's dictionary
 };
 
-// Work list of one kernel class, rebuilt every iteration by k_plan:
-// work[j] = affected tiles of jobs before j (work[njobs] = total),
-// first[j] = first affected tile of job j.
+// One contiguous run of tiles of one job to re-evaluate; cells outside
+// [tlo, thi] (and, for atom items, outside the reference's overlap test) are
+// left alone.
+struct WorkItem {
+    int job;
+    int tile;                 // first tile
+    int tiles;
+    int tlo, thi;             // time-index range
+    int atom;                 // 1: apply |t - tA| <= 3 (s + sA) per cell
+    int region;               // region index (proposal items), else -1
+};
+
+// Work list of one kernel class, rebuilt every iteration:
+// work[i] = tiles of items before i (work[*count] = total).
 struct ClassDev {
     const Job* jobs;
+    WorkItem* items;
     int* work;
-    int* first;
+    int* count;               // items in the list
     int njobs;
+    int max_items;
     int ncell;                // cells per workgroup of the class
+    int cls;
+};
+
+// time-index range of one (scale, 256-sample block) region of a dictionary
+struct RegionDev {
+    int tlo, thi;             // thi < tlo: no column of the scale falls in the block
 };
 
 // per (unit, scale): where the selection finds the scale's tile bests
@@ -92,6 +116,8 @@ struct UnitScale {
     long long tile_off;
     int tiles;
     int ncell;
+    int cls_slot;             // ClassDev slot of the scale's kernel class
+    int job;                  // job index inside that class
 };
 
 }  // namespace empi

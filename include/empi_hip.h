@@ -158,6 +158,29 @@ empi_status empi_plan_kernel_stats(const empi_plan* plan, int cls, int* sub_leng
 
 void        empi_plan_destroy(empi_plan* plan);
 
+/* ------------------------------------------------------------ learned proposal */
+
+/* Region-scoring network (no counterpart in the reference): a 3-layer
+ * perceptron input_dim -> 256 -> 256 -> 1 with ReLU, FP16 weights and FP32
+ * accumulation, trained on the device with Adam on a mean-squared-error loss.
+ * Parameters are one FP32 block: W1 [256][input_dim], b1 [256],
+ * W2 [256][256], b2 [256], w3 [256], b3 [1]; passes use their FP16 copies. */
+typedef struct empi_mlp empi_mlp;
+empi_status empi_mlp_create(int input_dim, unsigned seed, empi_mlp** out);
+void        empi_mlp_destroy(empi_mlp* mlp);
+size_t      empi_mlp_param_count(const empi_mlp* mlp);
+empi_status empi_mlp_get_params(const empi_mlp* mlp, float* out);
+empi_status empi_mlp_set_params(empi_mlp* mlp, const float* in);
+/* y[rows] from X[rows][input_dim]; both on the device when on_device != 0. */
+empi_status empi_mlp_forward(empi_mlp* mlp, const float* X, int rows, float* y, int on_device);
+/* Gradients of the mean squared error over one batch (host arrays), in the
+ * parameter layout; *loss receives the mean squared error. */
+empi_status empi_mlp_gradients(empi_mlp* mlp, const float* X, const float* target, int rows, float* grads,
+                               float* loss);
+/* Adam training over shuffled minibatches of contiguous rows. */
+empi_status empi_mlp_train(empi_mlp* mlp, const float* X, const float* target, int rows, int on_device,
+                           int epochs, int batch, float lr, unsigned seed, float* final_loss);
+
 /* Camel-case reference error text for a status code. */
 const char* empi_strerror(int status);
 
