@@ -34,7 +34,11 @@ EXPORTS = [
     "empi_plan_kernel_stats", "empi_plan_destroy", "empi_strerror", "empi_version",
     "empi_mlp_create", "empi_mlp_destroy", "empi_mlp_param_count", "empi_mlp_get_params",
     "empi_mlp_set_params", "empi_mlp_forward", "empi_mlp_gradients", "empi_mlp_train",
+    "empi_proposal_feature_dim", "empi_proposal_bound_factor", "empi_plan_set_proposal", "empi_plan_proposal_stats",
+    "empi_plan_set_training_capacity", "empi_plan_training_rows",
 ]
+
+PROPOSAL_OFF, PROPOSAL_ON, PROPOSAL_COLLECT = 0, 1, 2
 
 
 class DictParams(C.Structure):
@@ -113,6 +117,13 @@ def lib():
         L.empi_mlp_forward.argtypes = [vp, vp, C.c_int, vp, C.c_int]
         L.empi_mlp_gradients.argtypes = [vp, fp, fp, C.c_int, fp, fp]
         L.empi_mlp_train.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, fp]
+        L.empi_proposal_feature_dim.restype = C.c_int
+        L.empi_proposal_bound_factor.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.empi_proposal_bound_factor.restype = C.c_double
+        L.empi_plan_set_proposal.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double]
+        L.empi_plan_proposal_stats.argtypes = [vp, C.POINTER(C.c_ulonglong)]
+        L.empi_plan_set_training_capacity.argtypes = [vp, C.c_size_t]
+        L.empi_plan_training_rows.argtypes = [vp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]
         L.empi_strerror.argtypes = [C.c_int]
         L.empi_strerror.restype = C.c_char_p
         L.empi_version.restype = C.c_char_p
@@ -224,6 +235,36 @@ class Plan:
                 out.append({"class": c, "sub_length": q.value, "launches": n.value,
                             "total_ms": ms.value, "entries": e.value, "entry_channels": ec.value})
         return out
+
+    def set_proposal(self, mode=PROPOSAL_ON, net=None, top_k=8, delta=1e-3):
+        """Learned proposal (see empi_plan_set_proposal); net: an MLP or None.
+        The plan keeps a reference to the network."""
+        self._net = net
+        _check(lib().empi_plan_set_proposal(self._h, int(mode), net._h if net is not None else None,
+                                            int(top_k), float(delta)), "empi_plan_set_proposal")
+
+    def proposal_stats(self):
+        out = (C.c_ulonglong * 5)()
+        _check(lib().empi_plan_proposal_stats(self._h, out), "empi_plan_proposal_stats")
+        keys = ["fallback_iterations", "fallback_regions", "certified_iterations", "dirty_checked",
+                "proposed_regions"]
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def set_training_capacity(self, rows):
+        _check(lib().empi_plan_set_training_capacity(self._h, int(rows)), "empi_plan_set_training_capacity")
+        self._train_cap = int(rows)
+
+    def training_rows(self):
+        """(X [rows][feature_dim], target [rows]) collected since the last read."""
+        cap = getattr(self, "_train_cap", 0)
+        d = lib().empi_proposal_feature_dim()
+        X = np.zeros((cap, d), dtype=np.float32)
+        t = np.zeros(cap, dtype=np.float32)
+        n = C.c_size_t()
+        _check(lib().empi_plan_training_rows(self._h, X.ctypes.data_as(C.POINTER(C.c_float)),
+                                             t.ctypes.data_as(C.POINTER(C.c_float)), cap, C.byref(n)),
+               "empi_plan_training_rows")
+        return X[:n.value], t[:n.value]
 
     def close(self):
         if getattr(self, "_h", None):
@@ -354,6 +395,16 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+
+def proposal_bound_factor(scale, fs, nfft):
+    """c in the certified bound P <= c * window energy (host-only)."""
+    return lib().empi_proposal_bound_factor(float(scale), float(fs), int(nfft))
+
+
+def feature_dim():
+    """Input width of the proposal's region-scoring network."""
+    return lib().empi_proposal_feature_dim()
 
 
 def version():

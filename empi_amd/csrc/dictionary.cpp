@@ -87,4 +87,12 @@ empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictio
     return EMPI_OK;
 }
 
+double proposal_bound_factor(double s, double fs, int Nfft) {
+    const double sfs = s * fs;
+    const double a1 = sfs / Nfft;
+    const double den = -std::expm1(-2.0 * M_PI * a1 * a1);
+    const double alias = std::exp(-M_PI * sfs * sfs / 8.0) + 4.4 * std::exp(-M_PI * sfs * sfs / 2.0);
+    return (1.0 + alias / std::max(den, 1e-30)) * (1.0 + 1e-4);
+}
+
 }  // namespace empi

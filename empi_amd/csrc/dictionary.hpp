@@ -55,4 +55,15 @@ inline void column_window(double fs, int N, double tMax, int T, double s, int t,
 // Builds the dictionary; returns EMPI_OK, EMPI_INVALID_ARGUMENT or EMPI_TOO_LONG.
 empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictionary* out);
 
+// Factor c of the proposal's energy bound P <= c * (residual energy under the
+// column's window) for every cell of a scale.  With g the chosen real atom
+// (continuous unit norm), P = (sum_n x_n g_n)^2 / fs <= (window energy) *
+// sum_n g_n^2 / fs by Cauchy-Schwarz.  By Poisson summation the discrete norm
+// exceeds the continuous one, (s / 2 sqrt 2)(1 + cos(2 phi4) E(f4)), by at most
+// the aliases of the squared envelope at multiples of fs (2.2 e^{-pi (s fs)^2 / 2}
+// each side) and of its cos(2 omega t) part at fs - 2 f4 >= fs / 2
+// (e^{-pi (s fs)^2 / 8}); the continuous norm is at least 1 - E at bin 1 (bins 0
+// and Nyquist are real-valued, norm 2).  A 1e-4 margin covers FP32 rounding.
+double proposal_bound_factor(double s, double fs, int Nfft);
+
 }  // namespace empi

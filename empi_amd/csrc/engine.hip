@@ -28,6 +28,7 @@
 #include "dictionary.hpp"
 #include "empi_hip.h"
 #include "kernels.hpp"
+#include "mlp_internal.hpp"
 
 using namespace empi;
 
@@ -857,12 +858,12 @@ constexpr int kRegionThreads = 256;
 
 struct RegionState {
     float rmax;                      // exact max product at the last evaluation
-    float rbound;                    // certified bound on the current max (dirty)
+    float rbound;                    // bound on the current max (dirty regions)
     float rscore;                    // network score (predicted log max / residual)
     int flags;                       // bit 0 dirty, bit 1 evaluate now
     unsigned long long rbest;        // packed (product bits, ~bin) of the best cell
     int rcell;                       // best cell index within the scale
-    int pad;
+    float racc;                      // sqrt-product change bound accumulated since the evaluation
 };
 
 struct SelBest {
@@ -1001,12 +1002,13 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
 // Regions: (scale, 256-sample block) of one unit.  A region is clean while
 // every cell in it holds its exact product; subtracting an atom makes the
 // regions it overlaps (the reference's own rule) dirty.  Per iteration: the
-// network scores dirty regions, the top-K of them plus every dirty region of
-// a global scale (windows spanning a quarter of the signal) are evaluated
-// exactly, then every dirty region whose certified bound
-// csnorm * (residual energy under its windows) exceeds P* / (1 - delta) is
-// evaluated too (the full-search fallback), so the chosen atom is within
-// (1 - delta) of the exact maximum.
+// network scores the dirty regions and the top-K are evaluated exactly; then
+// every dirty region whose bound exceeds P* / (1 - delta), P* the best clean
+// product, is evaluated too (the exact fallback).  A region's bound is the
+// smaller of a Cauchy-Schwarz bound, csnorm * (residual energy under its
+// windows), which holds for every cell, and the stale maximum grown by the
+// accumulated atom-overlap bound (k_mark_dirty), which holds in the
+// continuous Gabor model.
 
 // affected time-index range of scale sd for the unit's last atom (widened by one)
 __device__ inline bool atom_range(const ScaleDev& sd, const UnitState& st, int* lo, int* hi) {
@@ -1065,11 +1067,24 @@ __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev
             R->rmax = __uint_as_float(static_cast<unsigned>(best >> 32));
             R->rcell = bcell;
             R->flags = 0;
+            R->racc = 0.0f;
         }
     }
 }
 
-// regions the unit's last atom overlaps become dirty
+// Regions the unit's last atom A overlaps (the reference's rule) become
+// dirty, and each accumulates a bound on how far A's subtraction can raise
+// the square root of any of its products: |<R - a g_A, g>| <= |<R, g>| +
+// |a| |<g_A, g>| for every unit real atom g of the region, with (continuous
+// Gabor atoms, n = the real-atom norms, I = the envelope overlap)
+//   |<g_A, g>| <= n_A n / 2 * I * (1 + e^{-pi fA^2 s^2 sA^2 / (s^2 + sA^2)}),
+//   I = s sA / sqrt(s^2 + sA^2) * e^{-pi dt^2 / (s^2 + sA^2)},
+// dt the distance from tA to the region's column centres, n = sqrt(2 sqrt 2 / s)
+// away from the lowest bins; 1% covers the discrete sums, and an additive
+// 2 (e^{-pi (s fs)^2 / 8} + e^{-pi (sA fs)^2 / 8}) their aliasing at scales of
+// a few samples.  The first bins, whose product normalisation depends on the
+// phase, can exceed this bound when a subtraction turns their phase; the
+// Cauchy-Schwarz bound holds there.
 __global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __restrict__ units,
                                                                const UnitState* __restrict__ state,
                                                                const ScaleDev* __restrict__ scales,
@@ -1077,22 +1092,41 @@ __global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __
                                                                RegionState* __restrict__ regions) {
     const int u = blockIdx.x;
     const UnitState st = state[u];
-    if (!st.active) return;
+    if (!st.active || st.iteration == 0) return;
     const UnitDev ud = units[u];
+    const empi_atom& a = ud.atoms[st.iteration - 1];
+    const double modulus = fabs(a.params[0]);
+    const double nA = modulus > 0.0 ? sqrt(ud.fs) * fabs(a.params[1]) / modulus : 0.0;
+    const double fA = a.params[4] * ud.fs * 0.5;
+    const double sA = st.sA;
+    const double coef = modulus * st.xscale;     // sqrt of the atom's product, FP32-map units
     for (int r = threadIdx.x; r < ud.L * ud.B; r += kRegionThreads) {
         const RegionDev rt = regtab[ud.regtab_off + r];
         if (rt.tlo > rt.thi) continue;
         const ScaleDev& sd = scales[ud.scale0 + r / ud.B];
         int lo, hi;
-        if (atom_range(sd, st, &lo, &hi) && rt.thi >= lo && rt.tlo <= hi) regions[ud.reg_off + r].flags |= 1;
+        if (!(atom_range(sd, st, &lo, &hi) && rt.thi >= lo && rt.tlo <= hi)) continue;
+        RegionState* R = &regions[ud.reg_off + r];
+        R->flags |= 1;
+        const double s = sd.s;
+        const double t0 = rt.tlo * sd.tMax / (sd.T - 1), t1 = rt.thi * sd.tMax / (sd.T - 1);
+        const double dt = st.tA < t0 ? t0 - st.tA : (st.tA > t1 ? st.tA - t1 : 0.0);
+        const double v = s * s + sA * sA;
+        const double overlap = s * sA / sqrt(v) * exp(-M_PI * dt * dt / v);
+        const double mirror = exp(-M_PI * fA * fA * s * s * sA * sA / v);
+        const double n = sqrt(2.0 * M_SQRT2 / s);
+        const double sfs = s * ud.fs, afs = sA * ud.fs;
+        const double alias = 2.0 * (exp(-M_PI * sfs * sfs / 8.0) + exp(-M_PI * afs * afs / 8.0));
+        const double rho = fmin(1.0, 1.01 * 0.5 * nA * n * overlap * (1.0 + mirror) + alias);
+        R->racc += static_cast<float>(coef * rho);
     }
 }
 
 // Prefix energies, block spectra of the blocks the last atom touched (all
-// blocks on init), certified bounds and network rows of dirty non-global
-// regions.  Row features: 64 pooled log powers of the 256-sample block, log
-// bound, log stale max, 8 scale features, block fill, log residual/total;
-// every energy relative to the current residual energy.
+// blocks on init), bounds and network rows of the dirty regions.  Row
+// features: 64 pooled log powers of the 256-sample block, log bound, log
+// stale max, 8 scale features, block fill, log residual/total; every energy
+// relative to the current residual energy.
 __global__ __launch_bounds__(kRegionThreads) void k_features(
     const UnitDev* __restrict__ units, const UnitState* __restrict__ state, const ScaleDev* __restrict__ scales,
     const RegionDev* __restrict__ regtab, const float* __restrict__ res32, RegionState* __restrict__ regions,
@@ -1162,14 +1196,15 @@ __global__ __launch_bounds__(kRegionThreads) void k_features(
         if (!(R->flags & 1)) continue;
         const int l = r / ud.B, b = r % ud.B;
         const ScaleDev& sd = scales[ud.scale0 + l];
-        if (sd.global) continue;
         const RegionDev rt = regtab[ud.regtab_off + r];
         // union of the windows of columns tlo..thi
         const double t0 = rt.tlo * sd.tMax / (sd.T - 1), t1 = rt.thi * sd.tMax / (sd.T - 1);
         const double hw = kGaussHalfWidth * sd.s;
         const long l0 = lrint((t0 - hw) * sd.fs), r1 = lrint((t1 + hw) * sd.fs);
         const int iL = static_cast<int>(l0 > 0 ? l0 : 0), iR = static_cast<int>(r1 < N - 1 ? r1 : N - 1);
-        const float bound = sd.csnorm * static_cast<float>(pre[iR + 1] - pre[iL]);
+        const float cs = sd.csnorm * static_cast<float>(pre[iR + 1] - pre[iL]);
+        const float grow = sqrtf(R->rmax) + R->racc;
+        const float bound = fminf(cs, grow * grow * 1.0001f);
         R->rbound = bound;
         const int row = atomicAdd(rowcount, 1);
         if (row >= max_rows) continue;
@@ -1217,7 +1252,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_choose(const UnitDev* __rest
     const int nreg = ud.L * ud.B;
     for (int r = tid; r < nreg; r += kRegionThreads) {
         RegionState* R = &regions[ud.reg_off + r];
-        if ((R->flags & 1) && (all || scales[ud.scale0 + r / ud.B].global)) R->flags |= 2;
+        if ((R->flags & 1) && all) R->flags |= 2;
     }
     if (all) return;
     __syncthreads();
@@ -1247,9 +1282,11 @@ __global__ __launch_bounds__(kRegionThreads) void k_choose(const UnitDev* __rest
             }
             __syncthreads();
         }
-        if (tid == 0 && bi[0] >= 0) regions[ud.reg_off + bi[0]].flags |= 2;
+        // every thread reads the winner before the next round rewrites bi
+        const int chosen = bi[0];
+        if (tid == 0 && chosen >= 0) regions[ud.reg_off + chosen].flags |= 2;
         __syncthreads();
-        if (bi[0] < 0) break;
+        if (chosen < 0) break;
     }
 }
 
@@ -1266,12 +1303,14 @@ __global__ __launch_bounds__(kRegionThreads) void k_region_items(const ClassDev*
                                                                  const ScaleDev* __restrict__ scales,
                                                                  const UnitScale* __restrict__ uscales,
                                                                  const RegionDev* __restrict__ regtab,
-                                                                 const RegionState* __restrict__ regions) {
+                                                                 const RegionState* __restrict__ regions,
+                                                                 unsigned long long* __restrict__ stat) {
     const int u = blockIdx.x;
     if (!state[u].active) return;
     const UnitDev ud = units[u];
     for (int r = threadIdx.x; r < ud.L * ud.B; r += kRegionThreads) {
         if (!(regions[ud.reg_off + r].flags & 2)) continue;
+        if (stat) atomicAdd(stat, 1ull);
         const RegionDev rt = regtab[ud.regtab_off + r];
         if (rt.tlo > rt.thi) continue;
         const int l = r / ud.B;
@@ -1470,6 +1509,7 @@ struct ClassJobs {
     DevBuf<int> work;          // [max_items + 1]
     DevBuf<int> count;         // [1]
     int max_items = 0;         // jobs, or regions of the class in proposal mode
+    long long region_items = 0;
 };
 
 struct ItemInfo {
@@ -1553,6 +1593,34 @@ struct empi_plan {
     DevBuf<float> d_input;
     DevBuf<unsigned long long> d_counter;
 
+    // proposal mode (single-channel units)
+    std::vector<RegionDev> regtab_h;
+    std::vector<int> dict_regtab_off, dict_B;
+    long long nregions = 0;
+    int prop_mode = EMPI_PROPOSAL_OFF;
+    empi_mlp* net = nullptr;
+    int top_k = 8;
+    float delta = 1e-3f;
+    bool prop_ready = false;
+    int max_rows = 0;
+    DevBuf<RegionDev> d_regtab;
+    DevBuf<RegionState> d_regions;
+    DevBuf<double> d_prefix;
+    DevBuf<long long> d_prefix_off;
+    DevBuf<float> d_blockfeat;
+    DevBuf<long long> d_feat_off;
+    DevBuf<float> d_rows;
+    DevBuf<int> d_rowmap;
+    DevBuf<float> d_rowref;
+    DevBuf<int> d_rowcount;
+    DevBuf<float> d_scores;
+    DevBuf<unsigned long long> d_pstats;
+    unsigned long long h_pstats[5] = {};
+    DevBuf<float> d_trainX, d_trainT;
+    DevBuf<unsigned long long> d_traincount;
+    hipGraphExec_t prop_exec = nullptr;
+    hipGraph_t prop_graph = nullptr;
+
     // host copies of the last run's results
     std::vector<empi_atom> h_atoms;
     std::vector<double> h_energies;
@@ -1574,6 +1642,8 @@ struct empi_plan {
     ~empi_plan() {
         if (iter_exec) (void)hipGraphExecDestroy(iter_exec);
         if (iter_graph) (void)hipGraphDestroy(iter_graph);
+        if (prop_exec) (void)hipGraphExecDestroy(prop_exec);
+        if (prop_graph) (void)hipGraphDestroy(prop_graph);
         if (stream) (void)hipStreamDestroy(stream);
         for (auto& v : ev)
             for (auto& e : v) {
@@ -1744,12 +1814,37 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             }
             s.gA = static_cast<float>(M_PI / ((sc.s * d.fs) * (sc.s * d.fs)));
             s.Wmax = sc.Wmax;
+            {
+                const double sfs = sc.s * d.fs;
+                s.csnorm = static_cast<float>(proposal_bound_factor(sc.s, d.fs, sc.Nfft));
+                s.global = 6.0 * sfs >= 0.25 * d.N ? 1 : 0;
+            }
             maxNfft = std::max(maxNfft, sc.Nfft);
             p->scales_h.push_back(s);
         }
     }
     p->twN = maxNfft;
     p->lgTWN = lg2(maxNfft);
+
+    // region tables: time indices of each (scale, 256-sample block); column t
+    // sits at sample t (N-1)/(T-1) and belongs to block floor(of that / 256)
+    for (size_t di = 0; di < p->dicts.size(); ++di) {
+        const Dictionary& d = p->dicts[di];
+        const long long N = d.N;
+        const int B = static_cast<int>((N + kBlock - 1) / kBlock);
+        p->dict_regtab_off.push_back(static_cast<int>(p->regtab_h.size()));
+        p->dict_B.push_back(B);
+        for (const Scale& sc : d.scales) {
+            const long long T = sc.T;
+            auto first_t = [&](long long b) { return (b * kBlock * (T - 1) + (N - 2)) / (N - 1); };
+            for (int b = 0; b < B; ++b) {
+                RegionDev r;
+                r.tlo = static_cast<int>(std::min<long long>(first_t(b), T));
+                r.thi = static_cast<int>(std::min<long long>(first_t(b + 1) - 1, T - 1));
+                p->regtab_h.push_back(r);
+            }
+        }
+    }
 
     // jobs per class, tile bests per (unit, scale)
     long long tile_off = 0;
@@ -1758,6 +1853,10 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
         const int did = u.scale0;
         u.scale0 = p->dict_scale0[did];
         u.us_off = static_cast<long long>(p->uscales_h.size());
+        u.B = p->dict_B[did];
+        u.regtab_off = p->dict_regtab_off[did];
+        u.reg_off = p->nregions;
+        p->nregions += static_cast<long long>(u.L) * u.B;
         for (int l = 0; l < u.L; ++l) {
             const ScaleDev& s = p->scales_h[u.scale0 + l];
             const int cls = s.lgQ - kMinLgQ;
@@ -1773,11 +1872,14 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             tile_off += j.tiles;
             if (cj.tiles + j.tiles > INT_MAX) return EMPI_UNSUPPORTED;
             cj.tiles += j.tiles;
-            cj.jobs.push_back(j);
             UnitScale us{};
             us.tile_off = j.tile_off;
             us.tiles = j.tiles;
             us.ncell = cpw;
+            us.cls_slot = mode * kNumClasses + cls;   // patched to the ClassDev slot below
+            us.job = static_cast<int>(cj.jobs.size());
+            cj.jobs.push_back(j);
+            cj.region_items += u.B;
             p->uscales_h.push_back(us);
         }
     }
@@ -1816,7 +1918,6 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_units.upload(p->units.data(), p->units.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_tw.upload(tw.data(), tw.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_etab.upload(etab_h.data(), etab_h.size(), s0)) != EMPI_OK) return st;
-    if ((st = p->d_uscales.upload(p->uscales_h.data(), p->uscales_h.size(), s0)) != EMPI_OK) return st;
     std::vector<ClassDev> cds;
     for (int mode = 0; mode < 3; ++mode) {
         for (int c = 0; c < kNumClasses; ++c) {
@@ -1824,7 +1925,8 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             if (cj.jobs.empty()) continue;
             if ((st = cj.d.alloc(cj.jobs.size())) != EMPI_OK) return st;
             if ((st = cj.d.upload(cj.jobs.data(), cj.jobs.size(), s0)) != EMPI_OK) return st;
-            cj.max_items = std::max<int>(static_cast<int>(cj.jobs.size()), cj.max_items);
+            if (cj.region_items > INT_MAX / 2) return EMPI_UNSUPPORTED;
+            cj.max_items = std::max<int>(static_cast<int>(cj.jobs.size()), static_cast<int>(cj.region_items));
             if ((st = cj.items.alloc(static_cast<size_t>(cj.max_items))) != EMPI_OK) return st;
             if ((st = cj.work.alloc(static_cast<size_t>(cj.max_items) + 1)) != EMPI_OK) return st;
             if ((st = cj.count.alloc(1)) != EMPI_OK) return st;
@@ -1844,14 +1946,17 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     p->nclass_dev = static_cast<int>(cds.size());
     if ((st = p->d_classes.alloc(cds.size())) != EMPI_OK) return st;
     if ((st = p->d_classes.upload(cds.data(), cds.size(), s0)) != EMPI_OK) return st;
+    for (UnitScale& us : p->uscales_h)
+        us.cls_slot = p->classes[us.cls_slot / kNumClasses][us.cls_slot % kNumClasses].slot;
+    if ((st = p->d_uscales.upload(p->uscales_h.data(), p->uscales_h.size(), s0)) != EMPI_OK) return st;
+    if ((st = p->d_regtab.alloc(p->regtab_h.size())) != EMPI_OK) return st;
+    if ((st = p->d_regtab.upload(p->regtab_h.data(), p->regtab_h.size(), s0)) != EMPI_OK) return st;
     EMPI_HIP(hipStreamSynchronize(s0));
     return EMPI_OK;
 }
 
-empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
-    if (p->nclass_dev == 0) return EMPI_OK;
-    hipLaunchKernelGGL(k_plan, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p, p->d_state.p,
-                       p->d_scales.p, full);
+// column kernels of every class over the current work lists
+empi_status launch_class_columns(empi_plan* p, hipStream_t s, int full) {
     for (int mc = 0; mc < 3 * kNumClasses; ++mc) {
         const int mode = mc / kNumClasses, c = mc % kNumClasses;
         ClassJobs& cj = p->classes[mode][c];
@@ -1874,6 +1979,119 @@ empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
     }
     EMPI_HIP(hipGetLastError());
     return EMPI_OK;
+}
+
+empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
+    if (p->nclass_dev == 0) return EMPI_OK;
+    hipLaunchKernelGGL(k_plan, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p, p->d_state.p,
+                       p->d_scales.p, full);
+    return launch_class_columns(p, s, full);
+}
+
+// ------------------------------------------------------------ proposal host side
+
+empi_status proposal_prepare(empi_plan* p) {
+    if (p->prop_ready) return EMPI_OK;
+    for (const UnitDev& u : p->units)
+        if (u.C != 1 || u.constraint) return EMPI_UNSUPPORTED;
+    const size_t U = p->units.size();
+    std::vector<long long> prefix_off(U), feat_off(U);
+    long long pre = 0, feat = 0, rows = 0;
+    for (size_t ui = 0; ui < U; ++ui) {
+        const UnitDev& u = p->units[ui];
+        prefix_off[ui] = pre;
+        pre += u.N + 1;
+        feat_off[ui] = feat;
+        feat += static_cast<long long>(u.B) * kFeatBins;
+        for (int l = 0; l < u.L; ++l) {
+            for (int b = 0; b < u.B; ++b) {
+                const RegionDev& r = p->regtab_h[u.regtab_off + l * u.B + b];
+                rows += r.tlo <= r.thi;
+            }
+        }
+    }
+    if (rows > INT_MAX / kFeatDim) return EMPI_UNSUPPORTED;
+    p->max_rows = static_cast<int>(rows);
+    empi_status st;
+    hipStream_t s0 = nullptr;
+    if ((st = p->d_regions.alloc(static_cast<size_t>(p->nregions))) != EMPI_OK) return st;
+    if ((st = p->d_prefix.alloc(static_cast<size_t>(pre))) != EMPI_OK) return st;
+    if ((st = p->d_prefix_off.alloc(U)) != EMPI_OK) return st;
+    if ((st = p->d_blockfeat.alloc(static_cast<size_t>(feat))) != EMPI_OK) return st;
+    if ((st = p->d_feat_off.alloc(U)) != EMPI_OK) return st;
+    const size_t R = static_cast<size_t>(std::max(1, p->max_rows));
+    if ((st = p->d_rows.alloc(R * kFeatDim)) != EMPI_OK) return st;
+    if ((st = p->d_rowmap.alloc(R)) != EMPI_OK) return st;
+    if ((st = p->d_rowref.alloc(R)) != EMPI_OK) return st;
+    if ((st = p->d_scores.alloc(R)) != EMPI_OK) return st;
+    if ((st = p->d_rowcount.alloc(1)) != EMPI_OK) return st;
+    if ((st = p->d_pstats.alloc(5)) != EMPI_OK) return st;
+    if ((st = p->d_prefix_off.upload(prefix_off.data(), U, s0)) != EMPI_OK) return st;
+    if ((st = p->d_feat_off.upload(feat_off.data(), U, s0)) != EMPI_OK) return st;
+    EMPI_HIP(hipStreamSynchronize(s0));
+    p->prop_ready = true;
+    return EMPI_OK;
+}
+
+// regions flagged "evaluate": work lists, column kernels, region bests
+empi_status proposal_pass(empi_plan* p, hipStream_t s, unsigned long long* stat) {
+    const int U = static_cast<int>(p->units.size());
+    hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, p->d_classes.p, p->nclass_dev, nullptr);
+    hipLaunchKernelGGL(k_region_items, dim3(U), dim3(kRegionThreads), 0, s, p->d_classes.p, p->d_units.p,
+                       p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_regtab.p, p->d_regions.p, stat);
+    hipLaunchKernelGGL(k_scan_items, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p);
+    EMPI_HIP(hipGetLastError());
+    empi_status st = launch_class_columns(p, s, 0);
+    if (st != EMPI_OK) return st;
+    hipLaunchKernelGGL(k_regions_reduce, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
+                       p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_regions.p, 0);
+    EMPI_HIP(hipGetLastError());
+    return EMPI_OK;
+}
+
+empi_status launch_features(empi_plan* p, hipStream_t s, int init) {
+    const int U = static_cast<int>(p->units.size());
+    hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, p->d_classes.p, 0, p->d_rowcount.p);
+    hipLaunchKernelGGL(k_features, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_scales.p,
+                       p->d_regtab.p, p->d_res32.p, p->d_regions.p, p->d_prefix.p, p->d_prefix_off.p,
+                       p->d_blockfeat.p, p->d_feat_off.p, p->d_rows.p, p->d_rowmap.p, p->d_rowref.p,
+                       p->d_rowcount.p, p->max_rows, init);
+    EMPI_HIP(hipGetLastError());
+    return EMPI_OK;
+}
+
+// one proposal iteration: select + subtract among clean regions, mark,
+// features and bounds, scores, proposed pass, certification, fallback pass
+empi_status launch_proposal_iteration(empi_plan* p, hipStream_t s) {
+    const int U = static_cast<int>(p->units.size());
+    empi_status st;
+    hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
+                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3, p->d_regions.p);
+    hipLaunchKernelGGL(k_mark_dirty, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_scales.p,
+                       p->d_regtab.p, p->d_regions.p);
+    EMPI_HIP(hipGetLastError());
+    if ((st = launch_features(p, s, 0)) != EMPI_OK) return st;
+    const bool collect = p->prop_mode == EMPI_PROPOSAL_COLLECT;
+    const bool scored = !collect && p->net != nullptr;
+    if (scored) {
+        if ((st = empi_mlp_forward_counted(p->net, p->d_rows.p, p->d_rowcount.p, p->max_rows, p->d_scores.p, s)))
+            return st;
+        hipLaunchKernelGGL(k_scores_scatter, dim3(256), dim3(256), 0, s, p->d_scores.p, p->d_rowmap.p,
+                           p->d_rowcount.p, p->max_rows, p->d_regions.p);
+    }
+    hipLaunchKernelGGL(k_choose, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_scales.p,
+                       p->d_regions.p, p->top_k, scored ? 1 : 0, collect ? 1 : 0);
+    EMPI_HIP(hipGetLastError());
+    if ((st = proposal_pass(p, s, p->d_pstats.p + 4)) != EMPI_OK) return st;
+    if (collect && p->d_traincount.p) {
+        hipLaunchKernelGGL(k_collect_rows, dim3(256), dim3(256), 0, s, p->d_rows.p, p->d_rowmap.p, p->d_rowref.p,
+                           p->d_rowcount.p, p->max_rows, p->d_regions.p, p->d_trainX.p, p->d_trainT.p,
+                           p->d_traincount.p, static_cast<unsigned long long>(p->d_trainT.n));
+    }
+    hipLaunchKernelGGL(k_certify, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_regions.p,
+                       p->delta, p->d_pstats.p);
+    EMPI_HIP(hipGetLastError());
+    return proposal_pass(p, s, nullptr);
 }
 
 empi_status collect_profile(empi_plan* p) {
@@ -1921,7 +2139,25 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     if ((st = launch_all_columns(p, s, 1)) != EMPI_OK) return st;
     std::vector<UnitState> hs(U);
     const bool graph = !p->profiling && std::getenv("EMPI_NO_GRAPH") == nullptr;
-    if (graph && !p->iter_exec) {
+    const bool prop = p->prop_mode != EMPI_PROPOSAL_OFF;
+    if (prop) {
+        EMPI_HIP(hipMemsetAsync(p->d_pstats.p, 0, sizeof(unsigned long long) * 5, s));
+        hipLaunchKernelGGL(k_regions_reduce, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
+                           p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_regions.p, 1);
+        EMPI_HIP(hipGetLastError());
+        if ((st = launch_features(p, s, 1)) != EMPI_OK) return st;
+        if (graph && !p->prop_exec) {
+            EMPI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            empi_status cst = launch_proposal_iteration(p, s);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            if (cst != EMPI_OK) return cst;
+            if (e != hipSuccess) return EMPI_DEVICE;
+            p->prop_graph = g;
+            EMPI_HIP(hipGraphInstantiate(&p->prop_exec, g, nullptr, nullptr, 0));
+        }
+    }
+    if (graph && !prop && !p->iter_exec) {
         // capture one iteration; every kernel reads its per-iteration state from HBM
         EMPI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
@@ -1935,7 +2171,13 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
         EMPI_HIP(hipGraphInstantiate(&p->iter_exec, g, nullptr, nullptr, 0));
     }
     for (int it = 0; it < p->maxIter; ++it) {
-        if (graph) {
+        if (prop) {
+            if (graph) {
+                EMPI_HIP(hipGraphLaunch(p->prop_exec, s));
+            } else if ((st = launch_proposal_iteration(p, s)) != EMPI_OK) {
+                return st;
+            }
+        } else if (graph) {
             EMPI_HIP(hipGraphLaunch(p->iter_exec, s));
         } else {
             hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
@@ -1973,6 +2215,11 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     EMPI_HIP(hipMemcpyAsync(p->h_state.data(), p->d_state.p, sizeof(UnitState) * U, hipMemcpyDeviceToHost, s));
     EMPI_HIP(hipMemcpyAsync(p->h_counter, p->d_counter.p, sizeof(unsigned long long) * 2 * kNumClasses,
                             hipMemcpyDeviceToHost, s));
+    if (prop) {
+        EMPI_HIP(hipMemcpyAsync(p->h_pstats, p->d_pstats.p, sizeof(unsigned long long) * 5, hipMemcpyDeviceToHost, s));
+    } else {
+        std::fill(p->h_pstats, p->h_pstats + 5, 0ull);
+    }
     EMPI_HIP(hipStreamSynchronize(s));
     if (p->profiling) {
         empi_status pst = collect_profile(p);
@@ -2117,6 +2364,69 @@ empi_status empi_plan_kernel_stats(const empi_plan* p, int cls, int* sub_length,
 int empi_plan_kernel_classes(void) { return kNumClasses; }
 
 void empi_plan_destroy(empi_plan* plan) { delete plan; }
+
+double empi_proposal_bound_factor(double scale, double fs, int Nfft) {
+    if (!(scale > 0) || !(fs > 0) || Nfft < 2) return NAN;
+    return proposal_bound_factor(scale, fs, Nfft);
+}
+
+int empi_proposal_feature_dim(void) { return kFeatDim; }
+
+empi_status empi_plan_set_proposal(empi_plan* p, int mode, empi_mlp* net, int top_k, double delta) {
+    if (!p || mode < EMPI_PROPOSAL_OFF || mode > EMPI_PROPOSAL_COLLECT || top_k < 0 || !(delta >= 0.0) ||
+        !(delta < 1.0))
+        return EMPI_INVALID_ARGUMENT;
+    if (net && empi_mlp_input_dim(net) != kFeatDim) return EMPI_INVALID_ARGUMENT;
+    if (mode != EMPI_PROPOSAL_OFF) {
+        empi_status st = proposal_prepare(p);
+        if (st != EMPI_OK) return st;
+    }
+    if (p->prop_exec) (void)hipGraphExecDestroy(p->prop_exec);
+    if (p->prop_graph) (void)hipGraphDestroy(p->prop_graph);
+    p->prop_exec = nullptr;
+    p->prop_graph = nullptr;
+    p->prop_mode = mode;
+    p->net = net;
+    p->top_k = top_k;
+    p->delta = static_cast<float>(delta);
+    return EMPI_OK;
+}
+
+empi_status empi_plan_proposal_stats(const empi_plan* p, unsigned long long out[5]) {
+    if (!p || !out) return EMPI_INVALID_ARGUMENT;
+    std::copy(p->h_pstats, p->h_pstats + 5, out);
+    return EMPI_OK;
+}
+
+empi_status empi_plan_set_training_capacity(empi_plan* p, size_t rows) {
+    if (!p) return EMPI_INVALID_ARGUMENT;
+    empi_status st;
+    if ((st = p->d_trainX.alloc(rows * kFeatDim)) != EMPI_OK) return st;
+    if ((st = p->d_trainT.alloc(rows)) != EMPI_OK) return st;
+    if ((st = p->d_traincount.alloc(rows ? 1 : 0)) != EMPI_OK) return st;
+    if (rows) EMPI_HIP(hipMemset(p->d_traincount.p, 0, sizeof(unsigned long long)));
+    // the captured iteration holds the old pointers
+    if (p->prop_exec) (void)hipGraphExecDestroy(p->prop_exec);
+    if (p->prop_graph) (void)hipGraphDestroy(p->prop_graph);
+    p->prop_exec = nullptr;
+    p->prop_graph = nullptr;
+    return EMPI_OK;
+}
+
+empi_status empi_plan_training_rows(empi_plan* p, float* X, float* target, size_t capacity, size_t* rows) {
+    if (!p || !rows) return EMPI_INVALID_ARGUMENT;
+    *rows = 0;
+    if (!p->d_traincount.p) return EMPI_OK;
+    unsigned long long n = 0;
+    EMPI_HIP(hipMemcpy(&n, p->d_traincount.p, sizeof n, hipMemcpyDeviceToHost));
+    n = std::min<unsigned long long>(n, p->d_trainT.n);
+    n = std::min<unsigned long long>(n, capacity);
+    if (n && X) EMPI_HIP(hipMemcpy(X, p->d_trainX.p, sizeof(float) * n * kFeatDim, hipMemcpyDeviceToHost));
+    if (n && target) EMPI_HIP(hipMemcpy(target, p->d_trainT.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    EMPI_HIP(hipMemset(p->d_traincount.p, 0, sizeof(unsigned long long)));
+    *rows = static_cast<size_t>(n);
+    return EMPI_OK;
+}
 
 // ----------------------------------------------------------- workspace API
 

@@ -181,6 +181,45 @@ empi_status empi_mlp_gradients(empi_mlp* mlp, const float* X, const float* targe
 empi_status empi_mlp_train(empi_mlp* mlp, const float* X, const float* target, int rows, int on_device,
                            int epochs, int batch, float lr, unsigned seed, float* final_loss);
 
+/* Proposal mode of a plan (no counterpart in the reference).  A region is one
+ * scale x one 256-sample block of one signal.  Subtracting an atom marks the
+ * regions it overlaps (the reference's |t - tA| <= 3 (s + sA) rule) dirty.
+ * Each iteration then evaluates exactly: every dirty region of a scale whose
+ * windows span a quarter of the signal, the top_k other dirty regions by
+ * network score (by their last exact maximum when net is NULL), and, as the
+ * full-search fallback, every dirty region whose certified bound
+ * csnorm * (residual energy under its windows) exceeds P / (1 - delta), P the
+ * best evaluated product.  The chosen atom is therefore within (1 - delta) of
+ * the exact maximum.  Single-channel units only (SMP, MMP2 sums, one-channel
+ * items); EMPI_UNSUPPORTED otherwise.
+ *   mode EMPI_PROPOSAL_OFF      the plain method (default)
+ *   mode EMPI_PROPOSAL_ON       as above
+ *   mode EMPI_PROPOSAL_COLLECT  every dirty region is evaluated (the plain
+ *                               method's atoms) and each dirty region's
+ *                               network input with its new exact maximum is
+ *                               recorded as a training row
+ * The network must outlive the plan's runs and have input dimension
+ * empi_proposal_feature_dim(). */
+#define EMPI_PROPOSAL_OFF 0
+#define EMPI_PROPOSAL_ON 1
+#define EMPI_PROPOSAL_COLLECT 2
+int         empi_proposal_feature_dim(void);
+/* Host-only: the factor c of the certified bound P <= c * (residual energy
+ * under the window) for a scale of s seconds with FFT length Nfft. */
+double      empi_proposal_bound_factor(double scale, double fs, int Nfft);
+empi_status empi_plan_set_proposal(empi_plan* plan, int mode, empi_mlp* net, int top_k, double delta);
+/* Counters of the last run: [0] unit-iterations that fell back to exact
+ * evaluation of violating regions, [1] violating regions evaluated,
+ * [2] certified unit-iterations, [3] dirty regions checked, [4] regions
+ * evaluated in the first (proposed) pass. */
+empi_status empi_plan_proposal_stats(const empi_plan* plan, unsigned long long out[5]);
+/* Training rows of COLLECT runs: capacity (rows kept on the device across
+ * runs until read), then a copy to host arrays X [rows][feature_dim],
+ * target [rows] (log of the region's exact maximum over the residual
+ * energy); *rows receives the number copied; reading resets the store. */
+empi_status empi_plan_set_training_capacity(empi_plan* plan, size_t rows);
+empi_status empi_plan_training_rows(empi_plan* plan, float* X, float* target, size_t capacity, size_t* rows);
+
 /* Camel-case reference error text for a status code. */
 const char* empi_strerror(int status);
 

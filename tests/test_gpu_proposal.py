@@ -1,0 +1,101 @@
+"""Learned proposal on the device (needs a GPU), judged by the exact search.
+
+* delta = 0: every iteration is certified exact, so the atoms must pass the
+  same FP64-oracle parity as the plain method.
+* COLLECT mode runs the plain method's atoms and records training rows; the
+  certified bound (feature 64, log bound / residual energy) must dominate
+  every row's new exact maximum (its target).
+* delta = 1e-3: explained-energy deficit against the plain run <= 0.1%,
+  with fewer map entries evaluated.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import empi_amd as E
+import pyoracle as po
+from empi_amd import synth
+from test_gpu_parity import _announce, _check_item, _oracle
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [1024, 1500, 2048, 3000]
+
+
+def _signals(seed0=4100):
+    return [synth.pink_with_atoms(seed0 + i, n, 256.0, 6) for i, n in enumerate(LENGTHS)]
+
+
+def _plan(iters=25):
+    return E.Plan([E.ItemSpec(1, n, 256.0, E.SMP, iters, 99.0, 0.01) for n in LENGTHS])
+
+
+def _flat(xs):
+    return np.concatenate([np.asarray(x, np.float32).ravel() for x in xs])
+
+
+def _explained(plan, n):
+    out = []
+    for i in range(n):
+        e = plan.result(i).energies[0]
+        e = e[~np.isnan(e)]
+        out.append(1.0 - e[-1] / e[0])
+    return np.array(out)
+
+
+def test_proposal_delta0_matches_oracle():
+    _announce("proposal delta=0 (no network), uneven batch of 4, 25 iterations")
+    xs = _signals()
+    plan = _plan()
+    plan.set_proposal(E.PROPOSAL_ON, None, top_k=8, delta=0.0)
+    plan.run(_flat(xs))
+    stats = plan.proposal_stats()
+    print("stats", stats, flush=True)
+    for i, x in enumerate(xs):
+        res = plan.result(i)
+        assert res.status == 0
+        ora = _oracle(x[None, :], 256.0, 0.01, po.SMP, 25, 99.0)
+        _check_item(res, ora, E.SMP, f"proposal item {i}")
+    assert stats["certified_iterations"] > 0
+
+
+def test_collect_rows_and_bound():
+    _announce("proposal COLLECT mode: rows, targets below the certified bound")
+    xs = _signals()
+    plan = _plan()
+    plan.set_proposal(E.PROPOSAL_COLLECT, None)
+    plan.set_training_capacity(200000)
+    plan.run(_flat(xs))
+    X, t = plan.training_rows()
+    print("rows", X.shape, flush=True)
+    assert X.shape[0] > 100 and X.shape[1] == E.feature_dim()
+    assert np.all(np.isfinite(X)) and np.all(np.isfinite(t))
+    # rows of regions whose new maximum is zero carry log(1e-30)
+    live = t > -60
+    assert np.all(t[live] <= X[live, 64] + 1e-3), float(np.max(t[live] - X[live, 64]))
+    for i, x in enumerate(xs):
+        res = plan.result(i)
+        ora = _oracle(x[None, :], 256.0, 0.01, po.SMP, 25, 99.0)
+        _check_item(res, ora, E.SMP, f"collect item {i}")
+    # reading resets the store
+    X2, _ = plan.training_rows()
+    assert X2.shape[0] == 0
+
+
+def test_proposal_delta_deficit_and_work():
+    _announce("proposal delta=1e-3 vs plain: explained-energy deficit and entries")
+    xs = _signals(4200)
+    plain = _plan(40)
+    plain.run(_flat(xs))
+    e_plain = _explained(plain, len(xs))
+    w_plain = plain.entries_evaluated()
+    prop = _plan(40)
+    prop.set_proposal(E.PROPOSAL_ON, None, top_k=8, delta=1e-3)
+    prop.run(_flat(xs))
+    e_prop = _explained(prop, len(xs))
+    w_prop = prop.entries_evaluated()
+    print("explained", e_plain, e_prop, "entries", w_plain, w_prop, prop.proposal_stats(), flush=True)
+    deficit = (e_plain - e_prop) / e_plain
+    assert np.all(deficit <= 1e-3), deficit
+    assert w_prop < w_plain
