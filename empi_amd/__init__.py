@@ -122,7 +122,7 @@ def lib():
         L.empi_proposal_bound_factor.restype = C.c_double
         L.empi_plan_set_proposal.argtypes = [vp, C.c_int, vp, C.c_int, C.c_double]
         L.empi_plan_proposal_stats.argtypes = [vp, C.POINTER(C.c_ulonglong)]
-        L.empi_plan_set_training_capacity.argtypes = [vp, C.c_size_t]
+        L.empi_plan_set_training_capacity.argtypes = [vp, C.c_size_t, C.c_uint]
         L.empi_plan_training_rows.argtypes = [vp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]
         L.empi_strerror.argtypes = [C.c_int]
         L.empi_strerror.restype = C.c_char_p
@@ -250,8 +250,9 @@ class Plan:
                 "proposed_regions"]
         return dict(zip(keys, [int(v) for v in out]))
 
-    def set_training_capacity(self, rows):
-        _check(lib().empi_plan_set_training_capacity(self._h, int(rows)), "empi_plan_set_training_capacity")
+    def set_training_capacity(self, rows, stride=1):
+        _check(lib().empi_plan_set_training_capacity(self._h, int(rows), int(stride)),
+               "empi_plan_set_training_capacity")
         self._train_cap = int(rows)
 
     def training_rows(self):

@@ -1030,12 +1030,13 @@ __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev
                                                                    const RegionDev* __restrict__ regtab,
                                                                    const unsigned long long* __restrict__ cellbest,
                                                                    RegionState* __restrict__ regions, int all) {
-    const int u = blockIdx.x;
+    const int u = blockIdx.y;
     if (!all && !state[u].active) return;
     const UnitDev ud = units[u];
+    const int waves = kRegionThreads / 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nreg = ud.L * ud.B;
-    for (int r = wave; r < nreg; r += kRegionThreads / 64) {
+    for (int r = blockIdx.x * waves + wave; r < nreg; r += gridDim.x * waves) {
         RegionState* R = &regions[ud.reg_off + r];
         if (!all && !(R->flags & 2)) continue;
         const int l = r / ud.B;
@@ -1122,28 +1123,22 @@ __global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __
     }
 }
 
-// Prefix energies, block spectra of the blocks the last atom touched (all
-// blocks on init), bounds and network rows of the dirty regions.  Row
-// features: 64 pooled log powers of the 256-sample block, log bound, log
-// stale max, 8 scale features, block fill, log residual/total; every energy
-// relative to the current residual energy.
-__global__ __launch_bounds__(kRegionThreads) void k_features(
-    const UnitDev* __restrict__ units, const UnitState* __restrict__ state, const ScaleDev* __restrict__ scales,
-    const RegionDev* __restrict__ regtab, const float* __restrict__ res32, RegionState* __restrict__ regions,
-    double* __restrict__ prefix, const long long* __restrict__ prefix_off, float* __restrict__ blockfeat,
-    const long long* __restrict__ feat_off, float* __restrict__ rows, int* __restrict__ rowmap,
-    float* __restrict__ rowref, int* __restrict__ rowcount, int max_rows, int init) {
+constexpr int kSlices = 32;          // workgroups per unit of the region kernels
+
+// prefix energies of the FP32 residual mirror, pre[n] = sum_{i<n} x_i^2 (FP64)
+__global__ __launch_bounds__(kRegionThreads) void k_prefix(const UnitDev* __restrict__ units,
+                                                           const UnitState* __restrict__ state,
+                                                           const float* __restrict__ res32,
+                                                           double* __restrict__ prefix,
+                                                           const long long* __restrict__ prefix_off) {
     __shared__ double part[kRegionThreads];
     const int u = blockIdx.x;
-    const UnitState st = state[u];
-    if (!st.active) return;
+    if (!state[u].active) return;
     const UnitDev ud = units[u];
     const int N = ud.N;
     const float* x = res32 + ud.res_off;
     double* pre = prefix + prefix_off[u];
-    float* bf = blockfeat + feat_off[u];
     const int tid = threadIdx.x;
-    // (a) prefix sums of x^2 (FP64), pre[n] = sum_{i<n} x_i^2
     const int per = (N + kRegionThreads - 1) / kRegionThreads;
     const int n0 = min(N, tid * per), n1 = min(N, n0 + per);
     double sum = 0.0;
@@ -1162,17 +1157,32 @@ __global__ __launch_bounds__(kRegionThreads) void k_features(
         run += static_cast<double>(x[n]) * x[n];
     }
     if (tid == kRegionThreads - 1) pre[N] = part[tid];
-    __syncthreads();
-    const double ref = pre[N] > 0.0 ? pre[N] : 1.0;
-    // (b) block spectra: 64 threads per block, bins 2m+1 and 2m+2 of a 256-point DFT
+}
+
+// log power spectra (64 pooled bins of a 256-point DFT) of the 256-sample
+// blocks the last atom touched, or of every block on init; one wave per block
+__global__ __launch_bounds__(kRegionThreads) void k_block_spectra(const UnitDev* __restrict__ units,
+                                                                  const UnitState* __restrict__ state,
+                                                                  const float* __restrict__ res32,
+                                                                  float* __restrict__ blockfeat,
+                                                                  const long long* __restrict__ feat_off,
+                                                                  int init) {
+    const int u = blockIdx.y;
+    const UnitState st = state[u];
+    if (!st.active) return;
+    const UnitDev ud = units[u];
+    const int N = ud.N;
+    const float* x = res32 + ud.res_off;
+    float* bf = blockfeat + feat_off[u];
     int b0 = 0, b1 = ud.B - 1;
     if (!init) {
         const double hw = kGaussHalfWidth * st.sA;
         b0 = max(0, static_cast<int>(lrint((st.tA - hw) * ud.fs)) / kBlock);
         b1 = min(ud.B - 1, static_cast<int>(lrint((st.tA + hw) * ud.fs)) / kBlock);
     }
-    for (int b = b0 + (tid >> 6); b <= b1; b += kRegionThreads / 64) {
-        const int m = tid & 63;
+    const int waves = kRegionThreads / 64;
+    const int m = threadIdx.x & 63;
+    for (int b = b0 + blockIdx.x * waves + (threadIdx.x >> 6); b <= b1; b += gridDim.x * waves) {
         float p = 0.0f;
         for (int q = 1; q <= 2; ++q) {
             const int k = 2 * m + q;
@@ -1186,12 +1196,31 @@ __global__ __launch_bounds__(kRegionThreads) void k_features(
             }
             p += re * re + im * im;
         }
-        bf[b * kFeatBins + m] = logf(1e-30f + p / static_cast<float>(ref));
+        bf[b * kFeatBins + m] = logf(1e-30f + p);
     }
-    __syncthreads();
-    // (c) bounds and rows of dirty regions on non-global scales
+}
+
+// Bounds and network rows of the dirty regions.  Row features: 64 pooled log
+// powers of the region's block, log bound, log stale max, 8 scale features,
+// block fill, log residual/total; every energy relative to the current
+// residual energy.
+__global__ __launch_bounds__(kRegionThreads) void k_rows(
+    const UnitDev* __restrict__ units, const UnitState* __restrict__ state, const ScaleDev* __restrict__ scales,
+    const RegionDev* __restrict__ regtab, RegionState* __restrict__ regions, const double* __restrict__ prefix,
+    const long long* __restrict__ prefix_off, const float* __restrict__ blockfeat,
+    const long long* __restrict__ feat_off, float* __restrict__ rows, int* __restrict__ rowmap,
+    float* __restrict__ rowref, int* __restrict__ rowcount, int max_rows) {
+    const int u = blockIdx.y;
+    const UnitState st = state[u];
+    if (!st.active) return;
+    const UnitDev ud = units[u];
+    const int N = ud.N;
+    const double* pre = prefix + prefix_off[u];
+    const float* bf = blockfeat + feat_off[u];
+    const double ref = pre[N] > 0.0 ? pre[N] : 1.0;
+    const float lref = logf(static_cast<float>(ref));
     const float eref = static_cast<float>(st.totalEnergy * ud.fs) * st.xscale * st.xscale;
-    for (int r = tid; r < ud.L * ud.B; r += kRegionThreads) {
+    for (int r = blockIdx.x * kRegionThreads + threadIdx.x; r < ud.L * ud.B; r += gridDim.x * kRegionThreads) {
         RegionState* R = &regions[ud.reg_off + r];
         if (!(R->flags & 1)) continue;
         const int l = r / ud.B, b = r % ud.B;
@@ -1209,7 +1238,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_features(
         const int row = atomicAdd(rowcount, 1);
         if (row >= max_rows) continue;
         float* f = rows + static_cast<long long>(row) * kFeatDim;
-        for (int q = 0; q < kFeatBins; ++q) f[q] = bf[b * kFeatBins + q];
+        for (int q = 0; q < kFeatBins; ++q) f[q] = bf[b * kFeatBins + q] - lref;
         f[64] = logf(1e-30f + bound / static_cast<float>(ref));
         f[65] = logf(1e-30f + R->rmax / static_cast<float>(ref));
         const float ls = log2f(sd.s * sd.fs) * (1.0f / 16.0f);
@@ -1377,14 +1406,21 @@ __global__ __launch_bounds__(kRegionThreads) void k_certify(const UnitDev* __res
     }
 }
 
-// training rows: targets log(new exact max / residual) of the rows k_features wrote
+// training rows: targets log(new exact max / residual) of the rows k_rows
+// wrote, one row in `stride` kept (by a hash of region and residual energy)
 __global__ void k_collect_rows(const float* __restrict__ rows, const int* __restrict__ rowmap,
                                const float* __restrict__ rowref, const int* __restrict__ rowcount, int max_rows,
                                const RegionState* __restrict__ regions, float* __restrict__ outX,
                                float* __restrict__ outT, unsigned long long* __restrict__ outCount,
-                               unsigned long long capacity) {
+                               unsigned long long capacity, unsigned stride) {
     const int n = min(*rowcount, max_rows);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (stride > 1) {
+            unsigned h = static_cast<unsigned>(rowmap[i]) * 2654435761u ^ __float_as_uint(rowref[i]) * 2246822519u;
+            h ^= h >> 15;
+            h *= 2654435761u;
+            if ((h >> 8) % stride != 0) continue;
+        }
         const unsigned long long pos = atomicAdd(outCount, 1ull);
         if (pos >= capacity) continue;
         for (int q = 0; q < kFeatDim; ++q) outX[pos * kFeatDim + q] = rows[static_cast<long long>(i) * kFeatDim + q];
@@ -1392,7 +1428,6 @@ __global__ void k_collect_rows(const float* __restrict__ rows, const int* __rest
     }
 }
 
-// MMP2 per-channel least-squares refit of the sum's atoms (mmp.cpp:36-44)
 template <int NT>
 __global__ __launch_bounds__(NT) void k_refit(const empi_atom* __restrict__ sum_atoms,
                                               const UnitState* __restrict__ sum_state,
@@ -1618,6 +1653,7 @@ struct empi_plan {
     unsigned long long h_pstats[5] = {};
     DevBuf<float> d_trainX, d_trainT;
     DevBuf<unsigned long long> d_traincount;
+    unsigned train_stride = 1;
     hipGraphExec_t prop_exec = nullptr;
     hipGraph_t prop_graph = nullptr;
 
@@ -2043,7 +2079,7 @@ empi_status proposal_pass(empi_plan* p, hipStream_t s, unsigned long long* stat)
     EMPI_HIP(hipGetLastError());
     empi_status st = launch_class_columns(p, s, 0);
     if (st != EMPI_OK) return st;
-    hipLaunchKernelGGL(k_regions_reduce, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
+    hipLaunchKernelGGL(k_regions_reduce, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
                        p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_regions.p, 0);
     EMPI_HIP(hipGetLastError());
     return EMPI_OK;
@@ -2052,10 +2088,14 @@ empi_status proposal_pass(empi_plan* p, hipStream_t s, unsigned long long* stat)
 empi_status launch_features(empi_plan* p, hipStream_t s, int init) {
     const int U = static_cast<int>(p->units.size());
     hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, p->d_classes.p, 0, p->d_rowcount.p);
-    hipLaunchKernelGGL(k_features, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_scales.p,
-                       p->d_regtab.p, p->d_res32.p, p->d_regions.p, p->d_prefix.p, p->d_prefix_off.p,
+    hipLaunchKernelGGL(k_prefix, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_res32.p,
+                       p->d_prefix.p, p->d_prefix_off.p);
+    hipLaunchKernelGGL(k_block_spectra, dim3(init ? kSlices : 4, U), dim3(kRegionThreads), 0, s, p->d_units.p,
+                       p->d_state.p, p->d_res32.p, p->d_blockfeat.p, p->d_feat_off.p, init);
+    hipLaunchKernelGGL(k_rows, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
+                       p->d_scales.p, p->d_regtab.p, p->d_regions.p, p->d_prefix.p, p->d_prefix_off.p,
                        p->d_blockfeat.p, p->d_feat_off.p, p->d_rows.p, p->d_rowmap.p, p->d_rowref.p,
-                       p->d_rowcount.p, p->max_rows, init);
+                       p->d_rowcount.p, p->max_rows);
     EMPI_HIP(hipGetLastError());
     return EMPI_OK;
 }
@@ -2086,7 +2126,7 @@ empi_status launch_proposal_iteration(empi_plan* p, hipStream_t s) {
     if (collect && p->d_traincount.p) {
         hipLaunchKernelGGL(k_collect_rows, dim3(256), dim3(256), 0, s, p->d_rows.p, p->d_rowmap.p, p->d_rowref.p,
                            p->d_rowcount.p, p->max_rows, p->d_regions.p, p->d_trainX.p, p->d_trainT.p,
-                           p->d_traincount.p, static_cast<unsigned long long>(p->d_trainT.n));
+                           p->d_traincount.p, static_cast<unsigned long long>(p->d_trainT.n), p->train_stride);
     }
     hipLaunchKernelGGL(k_certify, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_regions.p,
                        p->delta, p->d_pstats.p);
@@ -2142,8 +2182,8 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     const bool prop = p->prop_mode != EMPI_PROPOSAL_OFF;
     if (prop) {
         EMPI_HIP(hipMemsetAsync(p->d_pstats.p, 0, sizeof(unsigned long long) * 5, s));
-        hipLaunchKernelGGL(k_regions_reduce, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
-                           p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_regions.p, 1);
+        hipLaunchKernelGGL(k_regions_reduce, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p,
+                           p->d_state.p, p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_regions.p, 1);
         EMPI_HIP(hipGetLastError());
         if ((st = launch_features(p, s, 1)) != EMPI_OK) return st;
         if (graph && !p->prop_exec) {
@@ -2398,8 +2438,9 @@ empi_status empi_plan_proposal_stats(const empi_plan* p, unsigned long long out[
     return EMPI_OK;
 }
 
-empi_status empi_plan_set_training_capacity(empi_plan* p, size_t rows) {
-    if (!p) return EMPI_INVALID_ARGUMENT;
+empi_status empi_plan_set_training_capacity(empi_plan* p, size_t rows, unsigned stride) {
+    if (!p || stride == 0) return EMPI_INVALID_ARGUMENT;
+    p->train_stride = stride;
     empi_status st;
     if ((st = p->d_trainX.alloc(rows * kFeatDim)) != EMPI_OK) return st;
     if ((st = p->d_trainT.alloc(rows)) != EMPI_OK) return st;

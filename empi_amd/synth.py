@@ -78,11 +78,11 @@ def shared_multichannel(C=32, N=10240, fs=512.0, atoms=12, noise_sigma=0.2, seed
     return x.astype(np.float32)
 
 
-def uneven_batch(items=512, fs=256.0, nmin=1024, nmax=16384):
-    """Uneven lengths U{1024..16384}, pink noise + Poisson(6) atoms (seeds 4000+i)."""
+def uneven_batch(items=512, fs=256.0, nmin=1024, nmax=16384, seed0=4000):
+    """Uneven lengths U{1024..16384}, pink noise + Poisson(6) atoms (seeds seed0+i)."""
     out = []
     for i in range(items):
-        rng = np.random.default_rng(4000 + i)
+        rng = np.random.default_rng(seed0 + i)
         N = int(rng.integers(nmin, nmax + 1))
         count = int(rng.poisson(6))
         x = pink_noise(rng, N)

@@ -214,10 +214,11 @@ empi_status empi_plan_set_proposal(empi_plan* plan, int mode, empi_mlp* net, int
  * evaluated in the first (proposed) pass. */
 empi_status empi_plan_proposal_stats(const empi_plan* plan, unsigned long long out[5]);
 /* Training rows of COLLECT runs: capacity (rows kept on the device across
- * runs until read), then a copy to host arrays X [rows][feature_dim],
- * target [rows] (log of the region's exact maximum over the residual
- * energy); *rows receives the number copied; reading resets the store. */
-empi_status empi_plan_set_training_capacity(empi_plan* plan, size_t rows);
+ * runs until read) and sampling (one dirty region in stride, by a hash),
+ * then a copy to host arrays X [rows][feature_dim], target [rows] (log of
+ * the region's new exact maximum over the residual energy); *rows receives
+ * the number copied; reading resets the store. */
+empi_status empi_plan_set_training_capacity(empi_plan* plan, size_t rows, unsigned stride);
 empi_status empi_plan_training_rows(empi_plan* plan, float* X, float* target, size_t capacity, size_t* rows);
 
 /* Camel-case reference error text for a status code. */
