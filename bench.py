@@ -12,6 +12,10 @@ resident in HBM.  Other workloads:
   uneven   512 single-channel signals of 1024..16384 samples at 256 Hz, pink
            noise + Poisson(6) atoms, 99% / at most 100 iterations
 
+With the default workload the line also carries "proposal": the learned
+proposal against the plain method on the uneven batch (atoms/s of both,
+entries evaluated, explained-energy deficit per signal).
+
 Prints one JSON line on stdout (progress goes to stderr).  --gpus above 1 is
 refused: this build runs on one GPU.
 """
@@ -50,7 +54,63 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=20,
                     help="iterations of the CPU-baseline sample (smp-batch)")
+    ap.add_argument("--no-proposal", action="store_true",
+                    help="skip the learned-proposal comparison on the uneven batch")
+    ap.add_argument("--proposal-items", type=int, default=512)
+    ap.add_argument("--proposal-delta", type=float, default=1e-4)
     return ap.parse_args()
+
+
+def proposal_leg(args, torch):
+    """Learned proposal vs the plain method on the uneven batch (512 signals
+    of 1024..16384 samples at 256 Hz, pink noise + Poisson(6) atoms, 99% /
+    at most 100 iterations): atoms/s of both (one untimed run each first),
+    map entries evaluated, and the proposal's explained-energy deficit
+    against the plain method's, per signal."""
+    import empi_amd as E
+    from empi_amd import proposal, synth
+    xs = synth.uneven_batch(items=args.proposal_items)
+    specs = [E.ItemSpec(1, len(x), 256.0, E.SMP, 100, 99.0, 0.01) for x in xs]
+    dev = torch.from_numpy(np.concatenate([x.astype(np.float32) for x in xs])).to("cuda:0")
+    net = proposal.load()
+    out = {}
+    for name in ("plain", "proposal"):
+        plan = E.Plan(specs)
+        if name == "proposal":
+            plan.set_proposal(E.PROPOSAL_ON, net, top_k=8, delta=args.proposal_delta)
+        plan.run(device_ptr=dev.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run(device_ptr=dev.data_ptr())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        atoms = sum(int(plan.result(i).counts.sum()) for i in range(len(specs)))
+        expl = []
+        for i in range(len(specs)):
+            e = plan.result(i).energies[0]
+            e = e[~np.isnan(e)]
+            expl.append(1.0 - e[-1] / e[0])
+        out[name] = {"seconds": dt, "atoms": atoms, "atoms_per_s": atoms / dt,
+                     "entries": plan.entries_evaluated(), "explained": np.array(expl)}
+        if name == "proposal":
+            out[name]["stats"] = plan.proposal_stats()
+        plan.close()
+        progress(f"proposal leg: {name} {atoms / dt:.0f} atoms/s")
+    d = (out["plain"]["explained"] - out["proposal"]["explained"]) / out["plain"]["explained"]
+    st = out["proposal"]["stats"]
+    return {
+        "workload": (f"uneven batch: {len(specs)} single-channel signals of 1024..16384 samples at 256 Hz, "
+                     f"pink noise + Poisson(6) atoms, energyError 0.01, 99% / at most 100 iterations"),
+        "network": "empi_amd/proposal_net.npy" if net is not None else None,
+        "top_k": 8, "delta": args.proposal_delta,
+        "plain_atoms_per_s": out["plain"]["atoms_per_s"],
+        "proposal_atoms_per_s": out["proposal"]["atoms_per_s"],
+        "speedup": out["proposal"]["atoms_per_s"] / out["plain"]["atoms_per_s"],
+        "entries_ratio": out["plain"]["entries"] / out["proposal"]["entries"],
+        "explained_deficit_max": float(d.max()), "explained_deficit_mean": float(d.mean()),
+        "fallback_regions_per_unit_iteration": st["fallback_regions"] / max(1, st["certified_iterations"]),
+        "dirty_regions_per_unit_iteration": st["dirty_checked"] / max(1, st["certified_iterations"]),
+    }
 
 
 def cpu_sample(x, fs, eps, variant, iters, percent, limit, what):
@@ -199,6 +259,10 @@ def main():
                 "kernel": f"k_columns<{dom['sub_length']}>", "avg_launch_ms": avg_ms,
                 "entries_per_launch": dom["entries"] / dom["launches"]}
 
+    prop = None
+    if args.workload == "smp-batch" and not args.no_proposal:
+        prop = proposal_leg(args, torch)
+
     cpu_res = None
     if not args.no_cpu_baseline:
         progress("cpu baseline sample")
@@ -215,6 +279,7 @@ def main():
                    "mean_explained_energy": float(np.mean(explained))},
         "roofline": roofline,
         "cpu_baseline": cpu_res,
+        "proposal": prop,
         "kernels": stats,
     }
     print(json.dumps(out))

@@ -1200,16 +1200,43 @@ __global__ __launch_bounds__(kRegionThreads) void k_block_spectra(const UnitDev*
     }
 }
 
-// Bounds and network rows of the dirty regions.  Row features: 64 pooled log
-// powers of the region's block, log bound, log stale max, 8 scale features,
-// block fill, log residual/total; every energy relative to the current
-// residual energy.
+// best product over the unit's clean regions
+__global__ __launch_bounds__(kRegionThreads) void k_clean_best(const UnitDev* __restrict__ units,
+                                                               const UnitState* __restrict__ state,
+                                                               const RegionState* __restrict__ regions,
+                                                               float* __restrict__ pbest) {
+    __shared__ float red[kRegionThreads];
+    const int u = blockIdx.x;
+    if (!state[u].active) return;
+    const UnitDev ud = units[u];
+    const int tid = threadIdx.x;
+    float best = 0.0f;
+    for (int r = tid; r < ud.L * ud.B; r += kRegionThreads) {
+        const RegionState* R = &regions[ud.reg_off + r];
+        if (!(R->flags & 1)) best = fmaxf(best, R->rmax);
+    }
+    red[tid] = best;
+    __syncthreads();
+    for (int off = kRegionThreads / 2; off > 0; off >>= 1) {
+        if (tid < off) red[tid] = fmaxf(red[tid], red[tid + off]);
+        __syncthreads();
+    }
+    if (tid == 0) pbest[u] = red[0];
+}
+
+// Bounds of the dirty regions; those whose bound exceeds the best clean
+// product / (1 - delta) are candidates (flag bit 2) and get a network row
+// (every dirty region does when all_rows is set).  Row features: 64 pooled
+// log powers of the region's block, log bound, log stale max, 8 scale
+// features, block fill, log residual/total; every energy relative to the
+// current residual energy.
 __global__ __launch_bounds__(kRegionThreads) void k_rows(
     const UnitDev* __restrict__ units, const UnitState* __restrict__ state, const ScaleDev* __restrict__ scales,
     const RegionDev* __restrict__ regtab, RegionState* __restrict__ regions, const double* __restrict__ prefix,
     const long long* __restrict__ prefix_off, const float* __restrict__ blockfeat,
     const long long* __restrict__ feat_off, float* __restrict__ rows, int* __restrict__ rowmap,
-    float* __restrict__ rowref, int* __restrict__ rowcount, int max_rows) {
+    float* __restrict__ rowref, int* __restrict__ rowcount, int max_rows, const float* __restrict__ pbest,
+    float delta, int all_rows) {
     const int u = blockIdx.y;
     const UnitState st = state[u];
     if (!st.active) return;
@@ -1220,6 +1247,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_rows(
     const double ref = pre[N] > 0.0 ? pre[N] : 1.0;
     const float lref = logf(static_cast<float>(ref));
     const float eref = static_cast<float>(st.totalEnergy * ud.fs) * st.xscale * st.xscale;
+    const float limit = pbest[u] / (1.0f - delta);
     for (int r = blockIdx.x * kRegionThreads + threadIdx.x; r < ud.L * ud.B; r += gridDim.x * kRegionThreads) {
         RegionState* R = &regions[ud.reg_off + r];
         if (!(R->flags & 1)) continue;
@@ -1235,6 +1263,9 @@ __global__ __launch_bounds__(kRegionThreads) void k_rows(
         const float grow = sqrtf(R->rmax) + R->racc;
         const float bound = fminf(cs, grow * grow * 1.0001f);
         R->rbound = bound;
+        const bool candidate = bound > limit;
+        R->flags = candidate ? (R->flags | 4) : (R->flags & ~4);
+        if (!candidate && !all_rows) continue;
         const int row = atomicAdd(rowcount, 1);
         if (row >= max_rows) continue;
         float* f = rows + static_cast<long long>(row) * kFeatDim;
@@ -1290,7 +1321,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_choose(const UnitDev* __rest
         int bidx = -1;
         for (int r = tid; r < nreg; r += kRegionThreads) {
             const RegionState* R = &regions[ud.reg_off + r];
-            if ((R->flags & 3) != 1) continue;
+            if ((R->flags & 7) != 5) continue;   // dirty candidate, not chosen yet
             const float v = by_score ? R->rscore : R->rmax;
             if (v > best || (v == best && r < bidx)) {
                 best = v;
@@ -1650,6 +1681,7 @@ struct empi_plan {
     DevBuf<int> d_rowcount;
     DevBuf<float> d_scores;
     DevBuf<unsigned long long> d_pstats;
+    DevBuf<float> d_pbest;
     unsigned long long h_pstats[5] = {};
     DevBuf<float> d_trainX, d_trainT;
     DevBuf<unsigned long long> d_traincount;
@@ -2062,6 +2094,7 @@ empi_status proposal_prepare(empi_plan* p) {
     if ((st = p->d_scores.alloc(R)) != EMPI_OK) return st;
     if ((st = p->d_rowcount.alloc(1)) != EMPI_OK) return st;
     if ((st = p->d_pstats.alloc(5)) != EMPI_OK) return st;
+    if ((st = p->d_pbest.alloc(U)) != EMPI_OK) return st;
     if ((st = p->d_prefix_off.upload(prefix_off.data(), U, s0)) != EMPI_OK) return st;
     if ((st = p->d_feat_off.upload(feat_off.data(), U, s0)) != EMPI_OK) return st;
     EMPI_HIP(hipStreamSynchronize(s0));
@@ -2092,10 +2125,13 @@ empi_status launch_features(empi_plan* p, hipStream_t s, int init) {
                        p->d_prefix.p, p->d_prefix_off.p);
     hipLaunchKernelGGL(k_block_spectra, dim3(init ? kSlices : 4, U), dim3(kRegionThreads), 0, s, p->d_units.p,
                        p->d_state.p, p->d_res32.p, p->d_blockfeat.p, p->d_feat_off.p, init);
+    hipLaunchKernelGGL(k_clean_best, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
+                       p->d_regions.p, p->d_pbest.p);
     hipLaunchKernelGGL(k_rows, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
                        p->d_scales.p, p->d_regtab.p, p->d_regions.p, p->d_prefix.p, p->d_prefix_off.p,
                        p->d_blockfeat.p, p->d_feat_off.p, p->d_rows.p, p->d_rowmap.p, p->d_rowref.p,
-                       p->d_rowcount.p, p->max_rows);
+                       p->d_rowcount.p, p->max_rows, p->d_pbest.p, p->delta,
+                       p->prop_mode == EMPI_PROPOSAL_COLLECT ? 1 : 0);
     EMPI_HIP(hipGetLastError());
     return EMPI_OK;
 }

@@ -99,3 +99,34 @@ def test_proposal_delta_deficit_and_work():
     deficit = (e_plain - e_prop) / e_plain
     assert np.all(deficit <= 1e-3), deficit
     assert w_prop < w_plain
+
+
+def test_trained_network_gain():
+    """The committed network (trained on the device on plain-method atoms of
+    uneven batches with seeds 5000+i and 6000+i) against the method without
+    it, on 24 config-4 signals (seeds 4000+i): ranking the dirty regions by
+    the network, by their stale maximum, or not at all (top_k 0)."""
+    _announce("trained network vs stale ranking vs none, 24 uneven signals, 100 iterations")
+    from empi_amd import proposal
+    net = proposal.load()
+    assert net is not None, "empi_amd/proposal_net.npy missing"
+    xs = synth.uneven_batch(items=24)
+    specs = [E.ItemSpec(1, len(x), 256.0, E.SMP, 100, 99.0, 0.01) for x in xs]
+    flat = _flat(xs)
+    plain = E.Plan(specs)
+    plain.run(flat)
+    e_plain = _explained(plain, len(xs))
+    out = {"plain": plain.entries_evaluated()}
+    for name, k, n in [("none", 0, None), ("stale", 8, None), ("network", 8, net)]:
+        plan = E.Plan(specs)
+        plan.set_proposal(E.PROPOSAL_ON, n, top_k=k, delta=1e-4)
+        plan.run(flat)
+        deficit = (e_plain - _explained(plan, len(xs))) / e_plain
+        out[name] = plan.entries_evaluated()
+        print(name, f"entries {out[name]:.4g}", "stats", plan.proposal_stats(),
+              "deficit max", float(deficit.max()), flush=True)
+        assert float(deficit.max()) <= 1e-3, (name, deficit)
+        plan.close()
+    print("entries relative to plain:", {k: round(v / out["plain"], 4) for k, v in out.items()}, flush=True)
+    assert out["network"] < out["none"]
+    assert out["network"] < out["plain"]
