@@ -34,6 +34,7 @@ EXPORTS = [
     "empi_plan_kernel_stats", "empi_plan_destroy", "empi_strerror", "empi_version",
     "empi_mlp_create", "empi_mlp_destroy", "empi_mlp_param_count", "empi_mlp_get_params",
     "empi_mlp_set_params", "empi_mlp_forward", "empi_mlp_gradients", "empi_mlp_train",
+    "empi_plan_tuning_size", "empi_plan_autotune", "empi_plan_get_tuning", "empi_plan_set_tuning",
     "empi_proposal_feature_dim", "empi_proposal_bound_factor", "empi_plan_set_proposal", "empi_plan_proposal_stats",
     "empi_plan_set_training_capacity", "empi_plan_training_rows",
 ]
@@ -117,6 +118,10 @@ def lib():
         L.empi_mlp_forward.argtypes = [vp, vp, C.c_int, vp, C.c_int]
         L.empi_mlp_gradients.argtypes = [vp, fp, fp, C.c_int, fp, fp]
         L.empi_mlp_train.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, fp]
+        L.empi_plan_tuning_size.restype = C.c_int
+        L.empi_plan_autotune.argtypes = [vp, vp, C.c_int, C.c_int, vp]
+        L.empi_plan_get_tuning.argtypes = [vp, C.POINTER(C.c_int)]
+        L.empi_plan_set_tuning.argtypes = [vp, C.POINTER(C.c_int)]
         L.empi_proposal_feature_dim.restype = C.c_int
         L.empi_proposal_bound_factor.argtypes = [C.c_double, C.c_double, C.c_int]
         L.empi_proposal_bound_factor.restype = C.c_double
@@ -235,6 +240,27 @@ class Plan:
                 out.append({"class": c, "sub_length": q.value, "launches": n.value,
                             "total_ms": ms.value, "entries": e.value, "entry_channels": ec.value})
         return out
+
+    def autotune(self, signal=None, device_ptr=None, repeats=2):
+        """Times every column-kernel variant on this batch's full map build
+        and keeps the fastest per class; returns the tuning table."""
+        if device_ptr is not None:
+            status = lib().empi_plan_autotune(self._h, C.c_void_p(int(device_ptr)), 1, int(repeats), None)
+        else:
+            buf = np.ascontiguousarray(signal, dtype=np.float32).ravel()
+            status = lib().empi_plan_autotune(self._h, buf.ctypes.data_as(C.c_void_p), 0, int(repeats), None)
+        _check(status, "empi_plan_autotune")
+        return self.tuning()
+
+    def tuning(self):
+        out = np.zeros(lib().empi_plan_tuning_size(), dtype=np.int32)
+        _check(lib().empi_plan_get_tuning(self._h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    def set_tuning(self, table):
+        t = np.ascontiguousarray(table, dtype=np.int32)
+        assert t.size == lib().empi_plan_tuning_size()
+        _check(lib().empi_plan_set_tuning(self._h, t.ctypes.data_as(C.POINTER(C.c_int))), "empi_plan_set_tuning")
 
     def set_proposal(self, mode=PROPOSAL_ON, net=None, top_k=8, delta=1e-3):
         """Learned proposal (see empi_plan_set_proposal); net: an MLP or None.

@@ -133,6 +133,18 @@ enum { kModeSingle = 0, kModeSum = 1, kModeMmp1 = 2 };
 template <int Q, int MODE>
 constexpr int column_waves() { return MODE == kModeSingle ? Geo<Q>::WAVES : MODE == kModeSum ? 2 : 1; }
 
+// Tuning variants of the single-channel column kernels: waves per SIMD the
+// register allocator must allow (4, 3, 2), times two persistent grid sizes.
+// Q = 8192 fills a CU's LDS with one workgroup: always 2 waves.
+constexpr int kWaveVariants = 3;
+constexpr int kGridVariants = 2;
+constexpr int kTuneChoices = kWaveVariants * kGridVariants;
+template <int Q, int V>
+constexpr int variant_waves() { return Q >= 8192 ? 2 : (V == 0 ? 4 : V == 1 ? 3 : 2); }
+
+// E-table entries staged in LDS per work item (scales with more fall back to HBM)
+constexpr int kEtabLds = 128;
+
 // LDS index with one float2 of padding per 8: the stride-8 writes of the
 // first radix-8 passes then spread over all banks.
 __device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> 3); }
@@ -323,8 +335,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_scan_items(const ClassDev* __r
 // kModeSum: sum over channels of the products (MMP3); kModeMmp1: MMP1
 // constraint on low-frequency bins, channel sum on high ones.
 // Persistent: workgroups stride through the class's affected tiles.
-template <int Q, int MODE>
-__global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(column_waves<Q, MODE>()))) void k_columns(
+template <int Q, int MODE, int W>
+__global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W))) void k_columns(
     const ClassDev* __restrict__ classes, int cls, const ScaleDev* __restrict__ scales,
     const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
     const float* __restrict__ residual, unsigned long long* __restrict__ cellbest,
@@ -343,6 +355,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
     __shared__ unsigned long long best[NCELL];
     __shared__ unsigned long long tbest;
     __shared__ unsigned cnt_lds;
+    __shared__ float2 etl[kEtabLds + 1];    // ((1 - E)/K', 2E/K') of the item's scale; [kEtabLds] = (1/K', 0)
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -408,6 +421,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
             cnt_lds = 0;
             tbest = 0ull;
         }
+        for (int i = tid; i <= kEtabLds; i += NT) etl[i] = i < sd.JT ? sd.etab[i] : make_float2(sd.invKp, 0.0f);
         int any = 0;
         for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
         if (!__syncthreads_or(any)) continue;
@@ -466,14 +480,13 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
             for (int gi = 0; gi < EPT / GRP; ++gi) {
                 const int c = (tid + gi * GRP * NT) / (2 * Q);
                 const CellInfo ci = info[c];
-                float runp = 0.0f;
-                int runk = 0x7FFFFFFF;
+                unsigned long long run = 0ull;
                 auto element = [&](const int i) {
                     const int e = tid + i * NT;
                     const int rem = e % (2 * Q);
                     const int half = rem / Q, k1 = rem % Q;
                     const int k2 = half ? ci.kb : ci.ka;
-                    const int k = P * k1 + k2;
+                    const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
                     if (!(ci.valid && k2 >= 0 && k < F)) return;
                     const int slot = 2 * c + half;
                     const float2 Zk = buf[lpad(slot * Q + k1)];
@@ -487,8 +500,9 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
                     const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
                     const bool high = 4 * k > Nfft;
                     const int j = high ? M - k : k;
+                    // j, dint < 2^24: the 24-bit product is exact modulo 2^32, and Nfft divides 2^32
                     const unsigned ci_int =
-                        (static_cast<unsigned>(j) * static_cast<unsigned>(ci.dint)) & static_cast<unsigned>(Nfft - 1);
+                        __umul24(static_cast<unsigned>(j), static_cast<unsigned>(ci.dint)) & static_cast<unsigned>(Nfft - 1);
                     float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
                     if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
                     float sn, cs;
@@ -496,20 +510,14 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
                     // u = X e^{2 pi i cyc}; the product needs |u|^2 = |X|^2 and a = Re u
                     const float a = Xr * cs - Xi * sn;
                     const float num = Xr * Xr + Xi * Xi;
-                    float T1 = sd.invKp, T2 = 0.0f;
-                    if (j < sd.JT) {
-                        const float2 tt = sd.etab[j];
-                        T1 = tt.x;
-                        T2 = tt.y;
-                    }
+                    float2 tt = etl[min(j, kEtabLds)];
+                    if (sd.JT > kEtabLds && j >= kEtabLds && j < sd.JT) tt = sd.etab[j];
+                    const float T1 = tt.x, T2 = tt.y;
                     if constexpr (!MULTI) {
                         ++mycnt;
                         const float den = fmaf(T1, num, T2 * a * a);
                         const float pv = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
-                        if (pv > runp || (pv == runp && k < runk)) {
-                            runp = pv;
-                            runk = k;
-                        }
+                        run = max(run, pack_best(pv, static_cast<unsigned>(k)));
                     } else if (mmp1 && !high) {
                         if constexpr (mmp1) {
                             const float b = Xr * sn + Xi * cs;
@@ -531,7 +539,6 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
                 } else {
 #pragma unroll 2
                     for (int q = 0; q < GRP; ++q) element(gi * GRP + q);
-                    unsigned long long run = runk == 0x7FFFFFFF ? 0ull : pack_best(runp, static_cast<unsigned>(runk));
 #pragma unroll
                     for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
                     if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
@@ -555,8 +562,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
             for (int gi = 0; gi < EPT / GRP; ++gi) {
                 const int c = (tid + gi * GRP * NT) / (2 * Q);
                 const CellInfo ci = info[c];
-                float runp = 0.0f;
-                int runk = 0x7FFFFFFF;
+                unsigned long long run = 0ull;
 #pragma unroll
                 for (int q = 0; q < GRP; ++q) {
                     const int i = gi * GRP + q;
@@ -564,7 +570,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
                     const int rem = e % (2 * Q);
                     const int half = rem / Q, k1 = rem % Q;
                     const int k2 = half ? ci.kb : ci.ka;
-                    const int k = P * k1 + k2;
+                    const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
                     if (!(ci.valid && k2 >= 0 && k < F)) continue;
                     ++mycnt;
                     const bool high = 4 * k > Nfft;
@@ -580,23 +586,16 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(colu
                         const float lam = 0.5f * (Srr + Sii + Dm);
                         float cos2 = 1.0f;
                         if (Dm > 0.0f) cos2 = Dr >= 0.0f ? (Dm + Dr) / (2.0f * Dm) : (Di * Di) / (2.0f * Dm * (Dm - Dr));
-                        float T1 = sd.invKp, T2 = 0.0f;
-                        if (k < sd.JT) {
-                            const float2 tt = sd.etab[k];
-                            T1 = tt.x;
-                            T2 = tt.y;
-                        }
+                        float2 tt = etl[min(k, kEtabLds)];
+                        if (sd.JT > kEtabLds && k >= kEtabLds && k < sd.JT) tt = sd.etab[k];
+                        const float T1 = tt.x, T2 = tt.y;
                         const float den = fmaf(T2, cos2, T1);
                         pv = den > 0.0f ? lam / den : 0.0f;
                     } else {
                         pv = acc0[i];
                     }
-                    if (pv > runp || (pv == runp && k < runk)) {
-                        runp = pv;
-                        runk = k;
-                    }
+                    run = max(run, pack_best(pv, static_cast<unsigned>(k)));
                 }
-                unsigned long long run = runk == 0x7FFFFFFF ? 0ull : pack_best(runp, static_cast<unsigned>(runk));
 #pragma unroll
                 for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
                 if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
@@ -1597,29 +1596,56 @@ typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes,
                                unsigned long long* cellbest, unsigned long long* tilebest, const float2* tw,
                                int lgTWN, int full, unsigned long long* counter);
 
-template <int Q, int MODE>
+template <int Q, int MODE, int W>
 void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
                     const UnitDev* units, const UnitState* state, const float* residual,
                     unsigned long long* cellbest, unsigned long long* tilebest, const float2* tw, int lgTWN,
                     int full, unsigned long long* counter) {
-    hipLaunchKernelGGL((k_columns<Q, MODE>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
+    hipLaunchKernelGGL((k_columns<Q, MODE, W>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
                        state, residual, cellbest, tilebest, tw, lgTWN, full, counter);
 }
 
-// persistent grid: enough workgroups to fill 256 CUs a few times over
-// (a multiple of the 8 XCDs, which the kernel's work walk relies on)
-int column_grid(int Q, long long tiles) {
+// persistent grid: enough workgroups to fill 256 CUs a few times over (twice
+// that with grid variant 1), a multiple of the 8 XCDs, which the kernel's
+// work walk relies on
+int column_grid(int Q, long long tiles, int gv) {
     const int nt = Q >= 8192 ? (Q * 2) / 32 : Q >= 2048 ? (Q * 2) / 16 : 256;
-    const long long cap = nt >= 1024 ? 512 : nt >= 512 ? 1024 : 2048;
+    const long long cap = (nt >= 1024 ? 512 : nt >= 512 ? 1024 : 2048) << gv;
     const long long g = std::max(1ll, std::min(tiles, cap));
     return static_cast<int>((g + 7) / 8 * 8);
 }
 
-// [mode][class]
-const ColumnLauncher kLaunchers[3][kNumClasses] = {
-    {launch_columns<2, 0>, launch_columns<4, 0>, launch_columns<8, 0>, launch_columns<16, 0>, launch_columns<32, 0>, launch_columns<64, 0>, launch_columns<128, 0>, launch_columns<256, 0>, launch_columns<512, 0>, launch_columns<1024, 0>, launch_columns<2048, 0>, launch_columns<4096, 0>, launch_columns<8192, 0>},
-    {launch_columns<2, 1>, launch_columns<4, 1>, launch_columns<8, 1>, launch_columns<16, 1>, launch_columns<32, 1>, launch_columns<64, 1>, launch_columns<128, 1>, launch_columns<256, 1>, launch_columns<512, 1>, launch_columns<1024, 1>, launch_columns<2048, 1>, launch_columns<4096, 1>, launch_columns<8192, 1>},
-    {launch_columns<2, 2>, launch_columns<4, 2>, launch_columns<8, 2>, launch_columns<16, 2>, launch_columns<32, 2>, launch_columns<64, 2>, launch_columns<128, 2>, launch_columns<256, 2>, launch_columns<512, 2>, launch_columns<1024, 2>, launch_columns<2048, 2>, launch_columns<4096, 2>, launch_columns<8192, 2>}};
+#define EMPI_SINGLE_ROW(V)                                                                                   \
+    {launch_columns<2, 0, variant_waves<2, V>()>, launch_columns<4, 0, variant_waves<4, V>()>,               \
+     launch_columns<8, 0, variant_waves<8, V>()>, launch_columns<16, 0, variant_waves<16, V>()>,             \
+     launch_columns<32, 0, variant_waves<32, V>()>, launch_columns<64, 0, variant_waves<64, V>()>,           \
+     launch_columns<128, 0, variant_waves<128, V>()>, launch_columns<256, 0, variant_waves<256, V>()>,       \
+     launch_columns<512, 0, variant_waves<512, V>()>, launch_columns<1024, 0, variant_waves<1024, V>()>,     \
+     launch_columns<2048, 0, variant_waves<2048, V>()>, launch_columns<4096, 0, variant_waves<4096, V>()>,   \
+     launch_columns<8192, 0, variant_waves<8192, V>()>}
+#define EMPI_MULTI_ROW(M)                                                                                    \
+    {launch_columns<2, M, column_waves<2, M>()>, launch_columns<4, M, column_waves<4, M>()>,                 \
+     launch_columns<8, M, column_waves<8, M>()>, launch_columns<16, M, column_waves<16, M>()>,               \
+     launch_columns<32, M, column_waves<32, M>()>, launch_columns<64, M, column_waves<64, M>()>,             \
+     launch_columns<128, M, column_waves<128, M>()>, launch_columns<256, M, column_waves<256, M>()>,         \
+     launch_columns<512, M, column_waves<512, M>()>, launch_columns<1024, M, column_waves<1024, M>()>,       \
+     launch_columns<2048, M, column_waves<2048, M>()>, launch_columns<4096, M, column_waves<4096, M>()>,     \
+     launch_columns<8192, M, column_waves<8192, M>()>}
+
+// single-channel kernels [wave variant][class]; multi-channel [mode - 1][class]
+const ColumnLauncher kSingleLaunchers[kWaveVariants][kNumClasses] = {EMPI_SINGLE_ROW(0), EMPI_SINGLE_ROW(1),
+                                                                      EMPI_SINGLE_ROW(2)};
+const ColumnLauncher kMultiLaunchers[2][kNumClasses] = {EMPI_MULTI_ROW(1), EMPI_MULTI_ROW(2)};
+#undef EMPI_SINGLE_ROW
+#undef EMPI_MULTI_ROW
+
+ColumnLauncher column_launcher(int mode, int cls, int choice) {
+    return mode == kModeSingle ? kSingleLaunchers[choice % kWaveVariants][cls] : kMultiLaunchers[mode - 1][cls];
+}
+
+// default tuning: wave variant per class measured on the 64 x 20 x 5120 SMP
+// batch (grid variant 0)
+const int kDefaultTune[kNumClasses] = {0, 0, 2, 1, 0, 0, 2, 1, 1, 1, 1, 0, 0};
 
 int cells_per_wg(int Q) {
     const int nsl = Q >= 2048 ? 2 : std::min(512, 4096 / Q);
@@ -1635,6 +1661,7 @@ struct empi_plan {
     std::vector<UnitDev> units;
     std::vector<ScaleDev> scales_h;
     ClassJobs classes[3][kNumClasses];   // [mode][class]
+    int tune[3][kNumClasses];            // column-kernel variant per (mode, class): wave + 3 * grid variant
     int maxIter = 0;
     int twN = 2, lgTWN = 1;
     long long total_in = 0;
@@ -1707,6 +1734,18 @@ struct empi_plan {
     size_t ev_used[kNumClasses] = {};
     double class_ms[kNumClasses] = {};
     long long class_launches[kNumClasses] = {};
+    empi_plan() {
+        for (int m = 0; m < 3; ++m)
+            for (int c = 0; c < kNumClasses; ++c) tune[m][c] = m == kModeSingle ? kDefaultTune[c] : 0;
+    }
+    void drop_graphs() {
+        if (iter_exec) (void)hipGraphExecDestroy(iter_exec);
+        if (iter_graph) (void)hipGraphDestroy(iter_graph);
+        if (prop_exec) (void)hipGraphExecDestroy(prop_exec);
+        if (prop_graph) (void)hipGraphDestroy(prop_graph);
+        iter_exec = prop_exec = nullptr;
+        iter_graph = prop_graph = nullptr;
+    }
     ~empi_plan() {
         if (iter_exec) (void)hipGraphExecDestroy(iter_exec);
         if (iter_graph) (void)hipGraphDestroy(iter_graph);
@@ -2040,9 +2079,11 @@ empi_status launch_class_columns(empi_plan* p, hipStream_t s, int full) {
             evp = &p->ev[c][p->ev_used[c]++];
             EMPI_HIP(hipEventRecord(evp->first, s));
         }
-        kLaunchers[mode][c](column_grid(1 << (c + kMinLgQ), cj.tiles), s, p->d_classes.p, cj.slot, p->d_scales.p,
-                             p->d_units.p, p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p, p->d_tw.p,
-                             p->lgTWN, full, p->d_counter.p);
+        const int choice = p->tune[mode][c];
+        column_launcher(mode, c, choice)(column_grid(1 << (c + kMinLgQ), cj.tiles, choice / kWaveVariants), s,
+                                         p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p, p->d_state.p,
+                                         p->d_res32.p, p->d_cells.p, p->d_tiles.p, p->d_tw.p, p->lgTWN, full,
+                                         p->d_counter.p);
         if (evp) EMPI_HIP(hipEventRecord(evp->second, s));
     }
     EMPI_HIP(hipGetLastError());
@@ -2183,6 +2224,68 @@ empi_status collect_profile(empi_plan* p) {
         p->ev_used[c] = 0;
     }
     return EMPI_OK;
+}
+
+// Autotuner: times every variant of every column-kernel class on the full
+// map build of this plan's batch (HIP events on the plan's stream, best of
+// `reps`) and keeps the fastest per class.
+empi_status plan_autotune(empi_plan* p, const float* signal, int on_device, int reps, hipStream_t user) {
+    empi_status st;
+    EMPI_HIP(hipStreamSynchronize(user));
+    if (!p->stream) EMPI_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    hipStream_t s = p->stream;
+    const float* in = signal;
+    if (!on_device) {
+        if (!p->d_input.p || p->d_input.n < static_cast<size_t>(p->total_in)) {
+            if ((st = p->d_input.alloc(static_cast<size_t>(p->total_in))) != EMPI_OK) return st;
+        }
+        if ((st = p->d_input.upload(signal, static_cast<size_t>(p->total_in), s)) != EMPI_OK) return st;
+        in = p->d_input.p;
+    }
+    const int U = static_cast<int>(p->units.size());
+    hipLaunchKernelGGL(k_init, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p, in, p->d_res.p);
+    hipLaunchKernelGGL(k_mirror, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p, p->d_res.p,
+                       p->d_res32.p);
+    if (p->nclass_dev)
+        hipLaunchKernelGGL(k_plan, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p, p->d_state.p,
+                           p->d_scales.p, 1);
+    EMPI_HIP(hipGetLastError());
+    hipEvent_t e0, e1;
+    EMPI_HIP(hipEventCreate(&e0));
+    EMPI_HIP(hipEventCreate(&e1));
+    empi_status result = EMPI_OK;
+    for (int mode = 0; mode < 3 && result == EMPI_OK; ++mode) {
+        for (int c = 0; c < kNumClasses && result == EMPI_OK; ++c) {
+            ClassJobs& cj = p->classes[mode][c];
+            if (cj.jobs.empty() || cj.tiles == 0) continue;
+            const int n = mode == kModeSingle ? kTuneChoices : kGridVariants;
+            float best = INFINITY;
+            int best_choice = p->tune[mode][c];
+            for (int v = 0; v < n && result == EMPI_OK; ++v) {
+                const int choice = mode == kModeSingle ? v : v * kWaveVariants;
+                for (int r = 0; r < std::max(1, reps); ++r) {
+                    if (hipEventRecord(e0, s) != hipSuccess) result = EMPI_DEVICE;
+                    column_launcher(mode, c, choice)(column_grid(1 << (c + kMinLgQ), cj.tiles, choice / kWaveVariants),
+                                                     s, p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p,
+                                                     p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p,
+                                                     p->d_tw.p, p->lgTWN, 1, nullptr);
+                    float ms = 0.0f;
+                    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                        result = EMPI_DEVICE;
+                    if (result == EMPI_OK && ms < best) {
+                        best = ms;
+                        best_choice = choice;
+                    }
+                }
+            }
+            p->tune[mode][c] = best_choice;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->drop_graphs();
+    return result;
 }
 
 empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream_t user) {
@@ -2444,6 +2547,37 @@ void empi_plan_destroy(empi_plan* plan) { delete plan; }
 double empi_proposal_bound_factor(double scale, double fs, int Nfft) {
     if (!(scale > 0) || !(fs > 0) || Nfft < 2) return NAN;
     return proposal_bound_factor(scale, fs, Nfft);
+}
+
+int empi_plan_tuning_size(void) { return 3 * kNumClasses; }
+
+empi_status empi_plan_autotune(empi_plan* p, const float* signal, int signal_on_device, int repeats, void* stream) {
+    if (!p || !signal || repeats < 0) return EMPI_INVALID_ARGUMENT;
+    try {
+        return plan_autotune(p, signal, signal_on_device, repeats, static_cast<hipStream_t>(stream));
+    } catch (const std::bad_alloc&) {
+        return EMPI_NO_MEMORY;
+    }
+}
+
+empi_status empi_plan_get_tuning(const empi_plan* p, int* out) {
+    if (!p || !out) return EMPI_INVALID_ARGUMENT;
+    for (int m = 0; m < 3; ++m)
+        for (int c = 0; c < kNumClasses; ++c) out[m * kNumClasses + c] = p->tune[m][c];
+    return EMPI_OK;
+}
+
+empi_status empi_plan_set_tuning(empi_plan* p, const int* in) {
+    if (!p || !in) return EMPI_INVALID_ARGUMENT;
+    for (int m = 0; m < 3; ++m)
+        for (int c = 0; c < kNumClasses; ++c) {
+            const int v = in[m * kNumClasses + c];
+            if (v < 0 || v >= kTuneChoices || (m != kModeSingle && v % kWaveVariants != 0)) return EMPI_INVALID_ARGUMENT;
+        }
+    for (int m = 0; m < 3; ++m)
+        for (int c = 0; c < kNumClasses; ++c) p->tune[m][c] = in[m * kNumClasses + c];
+    p->drop_graphs();
+    return EMPI_OK;
 }
 
 int empi_proposal_feature_dim(void) { return kFeatDim; }

@@ -158,6 +158,19 @@ empi_status empi_plan_kernel_stats(const empi_plan* plan, int cls, int* sub_leng
 
 void        empi_plan_destroy(empi_plan* plan);
 
+/* Autotuner (no counterpart in the reference).  Each column-kernel class has
+ * variants: waves per SIMD the register allocator must allow (4, 3 or 2) for
+ * single-channel units, and two persistent grid sizes.  empi_plan_autotune
+ * times every variant of every class on the full map build of this batch
+ * (signal as for empi_plan_run; best of `repeats`, HIP events on the plan's
+ * stream) and keeps the fastest; the table, empi_plan_tuning_size() ints
+ * ([mode][class]: wave variant + 3 * grid variant), can be read and set. */
+int         empi_plan_tuning_size(void);
+empi_status empi_plan_autotune(empi_plan* plan, const float* signal, int signal_on_device, int repeats,
+                               void* stream);
+empi_status empi_plan_get_tuning(const empi_plan* plan, int* out);
+empi_status empi_plan_set_tuning(empi_plan* plan, const int* in);
+
 /* ------------------------------------------------------------ learned proposal */
 
 /* Region-scoring network (no counterpart in the reference): a 3-layer
