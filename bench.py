@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="smp-batch",
-                    choices=["smp-batch", "single", "mmp3", "uneven", "copy-calibration"])
+                    choices=["smp-batch", "single", "mmp3", "uneven", "proposal", "copy-calibration"])
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--channels", type=int, default=64)
     ap.add_argument("--iterations", type=int, default=None)
@@ -58,6 +58,8 @@ def parse():
                     help="skip the learned-proposal comparison on the uneven batch")
     ap.add_argument("--proposal-items", type=int, default=512)
     ap.add_argument("--proposal-delta", type=float, default=1e-4)
+    ap.add_argument("--tuning", default=None,
+                    help="comma-separated column-kernel tuning table instead of autotuning")
     return ap.parse_args()
 
 
@@ -108,6 +110,8 @@ def proposal_leg(args, torch):
                      f"pink noise + Poisson(6) atoms, energyError 0.01, 99% / at most 100 iterations"),
         "network": "empi_amd/proposal_net.npy" if net is not None else None,
         "top_k": 8, "delta": args.proposal_delta,
+        "plain_seconds": out["plain"]["seconds"], "proposal_seconds": out["proposal"]["seconds"],
+        "atoms": out["proposal"]["atoms"],
         "plain_atoms_per_s": out["plain"]["atoms_per_s"],
         "proposal_atoms_per_s": out["proposal"]["atoms_per_s"],
         "speedup": out["proposal"]["atoms_per_s"] / out["plain"]["atoms_per_s"],
@@ -191,6 +195,18 @@ def workload(args):
     return specs, [x[None, :] for x in xs], desc, cpu
 
 
+def measured_traffic(kernel_q, desc):
+    """HBM bytes per launch of the dominant column kernel from the committed
+    FETCH_SIZE / WRITE_SIZE summary of the same workload, else None."""
+    import glob
+    for path in glob.glob(os.path.join(ROOT, "profiles", "k_columns_*_traffic_*.json")):
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("sub_length") == kernel_q and d.get("workload") == desc:
+            return d["hbm_bytes_per_launch"], os.path.relpath(path, ROOT)
+    return None, None
+
+
 def copy_calibration():
     """One device-to-device copy of exactly 1 GiB: the reference reading for
     the FETCH_SIZE / WRITE_SIZE counters (expected 1048576 KiB each)."""
@@ -214,6 +230,16 @@ def main():
     import torch
     import empi_amd as E
 
+    if args.workload == "proposal":
+        prop = proposal_leg(args, torch)
+        print(json.dumps({
+            "metric": METRIC, "value": prop["proposal_atoms_per_s"], "unit": "atoms/s", "n_gpus": 1,
+            "steps": 1, "warmup": 1, "ms_per_step": prop["proposal_seconds"] * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32 maps / fp64 energy", "data": "synthetic",
+            "config": {"workload": prop["workload"], "method": "proposal"}, "proposal": prop}))
+        return
+
     specs, signals, desc, cpu = workload(args)
     plan = E.Plan(specs)
     flat = np.concatenate([np.asarray(s, np.float32).ravel() for s in signals])
@@ -222,9 +248,13 @@ def main():
     torch.cuda.synchronize()
     ptr = dev.data_ptr()
 
-    t_tune = time.perf_counter()
-    tuning = plan.autotune(device_ptr=ptr, repeats=2)
-    progress(f"autotuned column kernels in {time.perf_counter() - t_tune:.1f} s")
+    if args.tuning:
+        plan.set_tuning([int(v) for v in args.tuning.split(",")])
+        tuning = plan.tuning()
+    else:
+        t_tune = time.perf_counter()
+        tuning = plan.autotune(device_ptr=ptr, repeats=2)
+        progress(f"autotuned column kernels in {time.perf_counter() - t_tune:.1f} s")
 
     def step():
         plan.run(device_ptr=ptr)
@@ -263,8 +293,9 @@ def main():
     bytes_per_launch = (8 * dom["entry_channels"] + 4 * dom["entries"]) / dom["launches"]
     avg_ms = dom["total_ms"] / dom["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(dom["sub_length"], desc)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": f"k_columns<{dom['sub_length']}>", "avg_launch_ms": avg_ms,
                 "entries_per_launch": dom["entries"] / dom["launches"]}
 

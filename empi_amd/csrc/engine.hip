@@ -107,12 +107,6 @@ __device__ inline float2 cis_rev(float rev) {
     return make_float2(__builtin_amdgcn_cosf(rev), __builtin_amdgcn_sinf(rev));
 }
 
-// Occupancy hint for the column kernels (waves per SIMD the register
-// allocator must allow); a build-time tuning knob.
-#ifndef EMPI_COLUMN_WAVES
-#define EMPI_COLUMN_WAVES 4   // measured best on gfx950 (k_columns, workload 2)
-#endif
-
 // Workgroup geometry of class Q: NSL slots of Q complex points in LDS,
 // 16 points per thread.
 template <int Q>
@@ -122,16 +116,14 @@ struct Geo {
     static constexpr int EPT = Q >= 8192 ? 32 : 16;
     static constexpr int NT = NPTS / EPT;
     static constexpr int NCELL = NSL / 2;
-    // waves per SIMD the register allocator must allow (single-channel mode;
-    // LDS caps Q = 8192 at one workgroup, 8 waves, per CU anyway)
-    static constexpr int WAVES = Q >= 8192 ? 2 : EMPI_COLUMN_WAVES;
 };
 
 // column kernel modes: one channel; channel sum of products (MMP3 and
 // multichannel workspaces); MMP1 constraint (three accumulators per bin)
 enum { kModeSingle = 0, kModeSum = 1, kModeMmp1 = 2 };
+// waves per SIMD the register allocator must allow in the multichannel modes
 template <int Q, int MODE>
-constexpr int column_waves() { return MODE == kModeSingle ? Geo<Q>::WAVES : MODE == kModeSum ? 2 : 1; }
+constexpr int column_waves() { return MODE == kModeSum ? 2 : 1; }
 
 // Tuning variants of the single-channel column kernels: waves per SIMD the
 // register allocator must allow (4, 3, 2), times two persistent grid sizes.
