@@ -27,23 +27,28 @@ def half(a):
     return np.asarray(a, dtype=np.float32).astype(np.float16).astype(np.float64)
 
 
-def forward(p, X):
+def forward(p, X, rounded=True):
+    """rounded: weights through FP16 and biases through FP32, as the device
+    reads them; False gives the exact FP64 network (finite-difference checks)."""
     din = X.shape[1]
     W1, b1, W2, b2, w3, b3 = unpack(p, din)
-    W1, W2, w3 = half(W1), half(W2), half(w3)
-    b1 = b1.astype(np.float32).astype(np.float64)
-    b2 = b2.astype(np.float32).astype(np.float64)
+    if rounded:
+        W1, W2, w3 = half(W1), half(W2), half(w3)
+        b1 = b1.astype(np.float32).astype(np.float64)
+        b2 = b2.astype(np.float32).astype(np.float64)
+        b3 = np.float64(np.float32(b3))
     h1 = np.maximum(X @ W1.T + b1, 0.0)
     h2 = np.maximum(h1 @ W2.T + b2, 0.0)
-    return h2 @ w3 + np.float64(np.float32(b3)), h1, h2
+    return h2 @ w3 + b3, h1, h2
 
 
-def gradients(p, X, t):
-    """(flat gradient, loss) of mean((y - t)^2)."""
+def gradients(p, X, t, rounded=True):
+    """(flat gradient, loss) of mean((y - t)^2); with rounded weights the
+    rounding is passed straight through (the device's training rule)."""
     R, din = X.shape
     W1, b1, W2, b2, w3, b3 = unpack(p, din)
-    W2h, w3h = half(W2), half(w3)
-    y, h1, h2 = forward(p, X)
+    W2h, w3h = (half(W2), half(w3)) if rounded else (W2, w3)
+    y, h1, h2 = forward(p, X, rounded)
     dy = 2.0 * (y - t) / R
     dw3 = h2.T @ dy
     db3 = dy.sum()
