@@ -166,8 +166,12 @@ __device__ inline void stockham_pass(float2* buf, const float2* __restrict__ tw,
     for (int b = 0; b < PER; ++b) {
         const int bi = tid + b * NT;
         const int slot = bi / BPS, j = bi % BPS;
-        const int k = j & (Ns - 1);
+        int k = j & (Ns - 1);
         if constexpr (Ns > 1) {
+            // twiddles are recomputed per pass rather than hoisted out of the
+            // persistent loop: holding them for every pass costs more
+            // registers than the hardware sine/cosine costs cycles
+            asm volatile("" : "+v"(k));
             // w^r, w = e^{-2 pi i k / (Ns R)}, by products of the hardware w (depth <= 3)
             float2 w[R];
             w[1] = cis_rev(-static_cast<float>(k) * (1.0f / static_cast<float>(Ns * R)));
@@ -418,7 +422,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
         if (!__syncthreads_or(any)) continue;
 
-        const float invN = 1.0f / static_cast<float>(Nfft);
+        const float invN = ldexpf(1.0f, -sd.lgNfft);   // Nfft is a power of two
         const float invM = 2.0f * invN;
         constexpr bool mmp1 = MODE == kModeMmp1;
         const int C = MULTI ? ud.C : 1;
@@ -442,22 +446,30 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
 #pragma unroll 2
                 for (int q = 0; q < GL; ++q) {
                     const int m = (tid + (gi * GL + q) * NT) % Q;
+                    const int n0 = 2 * m, n1 = 2 * m + 1;
+                    // unconditional loads at a clamped index: no branch around
+                    // them, so the unrolled iterations' loads overlap
+                    const bool v0 = ci.valid && n0 < ci.W, v1 = ci.valid && n1 < ci.W;
+                    const float x0 = x[v0 ? ci.iL + n0 : 0];
+                    const float x1 = x[v1 ? ci.iL + n1 : 0];
                     float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
                     if (ci.valid) {
-                        const int n0 = 2 * m, n1 = 2 * m + 1;
                         float w0 = 0.f, w1 = 0.f;
-                        if (n0 < ci.W) {
+                        if (v0) {
                             const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
-                            w0 = x[ci.iL + n0] * __expf(-sd.gA * u * u);
+                            w0 = x0 * __expf(-sd.gA * u * u);
                         }
-                        if (n1 < ci.W) {
+                        if (v1) {
                             const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
-                            w1 = x[ci.iL + n1] * __expf(-sd.gA * u * u);
+                            w1 = x1 * __expf(-sd.gA * u * u);
                         }
                         const float2 z = make_float2(w0, w1);
                         // pre-twiddle w_M^{k2 m}, k2 m < M: exact revolution counts
-                        za = ci.ka == 0 ? z : cmul(z, cis_rev(-static_cast<float>(ci.ka * m) * invM));
-                        if (ci.kb >= 0) zb = cmul(z, cis_rev(-static_cast<float>(ci.kb * m) * invM));
+                        const unsigned um = static_cast<unsigned>(m);   // k2 m < M < 2^24
+                        za = ci.ka == 0 ? z
+                                        : cmul(z, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.ka), um)) * invM));
+                        if (ci.kb >= 0)
+                            zb = cmul(z, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.kb), um)) * invM));
                     }
                     buf[lpad((2 * c) * Q + m)] = za;
                     buf[lpad((2 * c + 1) * Q + m)] = zb;
@@ -492,34 +504,41 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
                     const bool high = 4 * k > Nfft;
                     const int j = high ? M - k : k;
-                    // j, dint < 2^24: the 24-bit product is exact modulo 2^32, and Nfft divides 2^32
-                    const unsigned ci_int =
-                        __umul24(static_cast<unsigned>(j), static_cast<unsigned>(ci.dint)) & static_cast<unsigned>(Nfft - 1);
-                    float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
-                    if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
-                    float sn, cs;
-                    twiddle_rev(cyc, &sn, &cs);
-                    // u = X e^{2 pi i cyc}; the product needs |u|^2 = |X|^2 and a = Re u
-                    const float a = Xr * cs - Xi * sn;
                     const float num = Xr * Xr + Xi * Xi;
-                    float2 tt = etl[min(j, kEtabLds)];
-                    if (sd.JT > kEtabLds && j >= kEtabLds && j < sd.JT) tt = sd.etab[j];
-                    const float T1 = tt.x, T2 = tt.y;
+                    // Beyond JT the window term E is zero and P = K' |X|^2: the
+                    // phase of u = X e^{2 pi i cyc} enters only the first bins
+                    // (and MMP1's constraint on the low half).
+                    const bool edge = j < sd.JT;
+                    float a = 0.0f, b = 0.0f;
+                    if (edge || (mmp1 && !high)) {
+                        // j, dint < 2^24: the 24-bit product is exact modulo 2^32, and Nfft divides 2^32
+                        const unsigned ci_int = __umul24(static_cast<unsigned>(j), static_cast<unsigned>(ci.dint)) &
+                                                static_cast<unsigned>(Nfft - 1);
+                        float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
+                        if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
+                        float sn, cs;
+                        twiddle_rev(cyc, &sn, &cs);
+                        a = Xr * cs - Xi * sn;
+                        b = Xr * sn + Xi * cs;
+                    }
+                    float pe = num * sd.Kp;   // the product where E = 0
+                    if (edge) {
+                        float2 tt = etl[min(j, kEtabLds)];
+                        if (j >= kEtabLds) tt = sd.etab[j];
+                        const float den = fmaf(tt.x, num, tt.y * a * a);
+                        pe = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
+                    }
                     if constexpr (!MULTI) {
                         ++mycnt;
-                        const float den = fmaf(T1, num, T2 * a * a);
-                        const float pv = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
-                        run = max(run, pack_best(pv, static_cast<unsigned>(k)));
+                        run = max(run, pack_best(pe, static_cast<unsigned>(k)));
                     } else if (mmp1 && !high) {
                         if constexpr (mmp1) {
-                            const float b = Xr * sn + Xi * cs;
                             acc0[i] += a * a;
                             acc1[i] += b * b;
                             acc2[i] += a * b;
                         }
                     } else {
-                        const float den = fmaf(T1, num, T2 * a * a);
-                        acc0[i] += den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
+                        acc0[i] += pe;
                     }
                 };
                 if constexpr (MULTI) {
