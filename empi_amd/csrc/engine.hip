@@ -113,7 +113,7 @@ template <int Q>
 struct Geo {
     static constexpr int NSL = Q >= 2048 ? 2 : (4096 / Q > 512 ? 512 : 4096 / Q);
     static constexpr int NPTS = NSL * Q;
-    static constexpr int EPT = Q >= 8192 ? 32 : 16;
+    static constexpr int EPT = 16;
     static constexpr int NT = NPTS / EPT;
     static constexpr int NCELL = NSL / 2;
 };
@@ -132,7 +132,7 @@ constexpr int kWaveVariants = 3;
 constexpr int kGridVariants = 2;
 constexpr int kTuneChoices = kWaveVariants * kGridVariants;
 template <int Q, int V>
-constexpr int variant_waves() { return Q >= 8192 ? 2 : (V == 0 ? 4 : V == 1 ? 3 : 2); }
+constexpr int variant_waves() { return Q >= 8192 ? 4 : (V == 0 ? 4 : V == 1 ? 3 : 2); }
 
 // E-table entries staged in LDS per work item (scales with more fall back to HBM)
 constexpr int kEtabLds = 128;
@@ -1620,7 +1620,7 @@ void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, c
 // that with grid variant 1), a multiple of the 8 XCDs, which the kernel's
 // work walk relies on
 int column_grid(int Q, long long tiles, int gv) {
-    const int nt = Q >= 8192 ? (Q * 2) / 32 : Q >= 2048 ? (Q * 2) / 16 : 256;
+    const int nt = Q >= 2048 ? (Q * 2) / 16 : 256;
     const long long cap = (nt >= 1024 ? 512 : nt >= 512 ? 1024 : 2048) << gv;
     const long long g = std::max(1ll, std::min(tiles, cap));
     return static_cast<int>((g + 7) / 8 * 8);
