@@ -479,80 +479,102 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             fft_passes<Q, NT, NPTS, 1>(buf, tw, lgTWN);
 
             // post: split the packed spectrum, phase-correct, product per bin.
-            // Elements of one group share a cell: its info is read once.
-#pragma unroll MULTI ? EPT / GRP : 1
-            for (int gi = 0; gi < EPT / GRP; ++gi) {
-                const int c = (tid + gi * GRP * NT) / (2 * Q);
-                const CellInfo ci = info[c];
-                unsigned long long run = 0ull;
-                auto element = [&](const int i) {
-                    const int e = tid + i * NT;
-                    const int rem = e % (2 * Q);
-                    const int half = rem / Q, k1 = rem % Q;
-                    const int k2 = half ? ci.kb : ci.ka;
-                    const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
-                    if (!(ci.valid && k2 >= 0 && k < F)) return;
-                    const int slot = 2 * c + half;
-                    const float2 Zk = buf[lpad(slot * Q + k1)];
-                    float2 Zp;
-                    if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
-                    else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
-                    const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
-                    const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
-                    const float2 wv = cis_rev(-static_cast<float>(k) * invN);
-                    const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
-                    const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
-                    const bool high = 4 * k > Nfft;
-                    const int j = high ? M - k : k;
-                    const float num = Xr * Xr + Xi * Xi;
-                    // Beyond JT the window term E is zero and P = K' |X|^2: the
-                    // phase of u = X e^{2 pi i cyc} enters only the first bins
-                    // (and MMP1's constraint on the low half).
-                    const bool edge = j < sd.JT;
-                    float a = 0.0f, b = 0.0f;
-                    if (edge || (mmp1 && !high)) {
-                        // j, dint < 2^24: the 24-bit product is exact modulo 2^32, and Nfft divides 2^32
-                        const unsigned ci_int = __umul24(static_cast<unsigned>(j), static_cast<unsigned>(ci.dint)) &
-                                                static_cast<unsigned>(Nfft - 1);
-                        float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
-                        if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
-                        float sn, cs;
-                        twiddle_rev(cyc, &sn, &cs);
-                        a = Xr * cs - Xi * sn;
-                        b = Xr * sn + Xi * cs;
-                    }
-                    float pe = num * sd.Kp;   // the product where E = 0
-                    if (edge) {
-                        float2 tt = etl[min(j, kEtabLds)];
-                        if (j >= kEtabLds) tt = sd.etab[j];
-                        const float den = fmaf(tt.x, num, tt.y * a * a);
-                        pe = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
-                    }
-                    if constexpr (!MULTI) {
-                        ++mycnt;
-                        run = max(run, pack_best(pe, static_cast<unsigned>(k)));
-                    } else if (mmp1 && !high) {
-                        if constexpr (mmp1) {
-                            acc0[i] += a * a;
-                            acc1[i] += b * b;
-                            acc2[i] += a * b;
-                        }
-                    } else {
-                        acc0[i] += pe;
-                    }
-                };
-                if constexpr (MULTI) {
-#pragma unroll
-                    for (int q = 0; q < GRP; ++q) {
-                        element(gi * GRP + q);
-                        __builtin_amdgcn_sched_barrier(0);   // keep per-element live ranges short
+            // element(rem, c, ci, i, run): bin `rem` (of 2Q) of cell c; i indexes
+            // the multichannel accumulators.
+            auto element = [&](const int rem, const int c, const CellInfo& ci, const int i,
+                               unsigned long long& run) {
+                const int half = rem / Q, k1 = rem % Q;
+                const int k2 = half ? ci.kb : ci.ka;
+                const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
+                if (!(ci.valid && k2 >= 0 && k < F)) return;
+                const int slot = 2 * c + half;
+                const float2 Zk = buf[lpad(slot * Q + k1)];
+                float2 Zp;
+                if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
+                else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
+                const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
+                const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
+                const float2 wv = cis_rev(-static_cast<float>(k) * invN);
+                const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
+                const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
+                const bool high = 4 * k > Nfft;
+                const int j = high ? M - k : k;
+                const float num = Xr * Xr + Xi * Xi;
+                // Beyond JT the window term E is zero and P = K' |X|^2: the
+                // phase of u = X e^{2 pi i cyc} enters only the first bins
+                // (and MMP1's constraint on the low half).
+                const bool edge = j < sd.JT;
+                float a = 0.0f, b = 0.0f;
+                if (edge || (mmp1 && !high)) {
+                    // j, dint < 2^24: the 24-bit product is exact modulo 2^32, and Nfft divides 2^32
+                    const unsigned ci_int = __umul24(static_cast<unsigned>(j), static_cast<unsigned>(ci.dint)) &
+                                            static_cast<unsigned>(Nfft - 1);
+                    float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
+                    if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
+                    float sn, cs;
+                    twiddle_rev(cyc, &sn, &cs);
+                    a = Xr * cs - Xi * sn;
+                    b = Xr * sn + Xi * cs;
+                }
+                float pe = num * sd.Kp;   // the product where E = 0
+                if (edge) {
+                    float2 tt = etl[min(j, kEtabLds)];
+                    if (j >= kEtabLds) tt = sd.etab[j];
+                    const float den = fmaf(tt.x, num, tt.y * a * a);
+                    pe = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
+                }
+                if constexpr (!MULTI) {
+                    ++mycnt;
+                    run = max(run, pack_best(pe, static_cast<unsigned>(k)));
+                } else if (mmp1 && !high) {
+                    if constexpr (mmp1) {
+                        acc0[i] += a * a;
+                        acc1[i] += b * b;
+                        acc2[i] += a * b;
                     }
                 } else {
+                    acc0[i] += pe;
+                }
+            };
+            constexpr bool CONTIG = !MULTI && Q >= 8 && Q <= 256;
+            if constexpr (CONTIG) {
+                // Small sub-transforms: a thread's EPT elements are consecutive
+                // bins of one cell, visited in a rotated order that spreads the
+                // LDS banks, so each cell needs one cross-lane reduction per tile.
+                constexpr int TPC = 2 * Q / EPT;   // threads per cell
+                const int c = tid / TPC;
+                const CellInfo ci = info[c];
+                const int r0 = (tid % TPC) * EPT;
+                unsigned long long run = 0ull;
 #pragma unroll 2
-                    for (int q = 0; q < GRP; ++q) element(gi * GRP + q);
+                for (int s2 = 0; s2 < EPT; ++s2) element(r0 + ((s2 + tid) & (EPT - 1)), c, ci, 0, run);
 #pragma unroll
-                    for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
-                    if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
+                for (int off = TPC / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
+                if ((tid & (TPC - 1)) == 0 && run) atomicMax(&best[c], run);
+            } else {
+                // elements of one group share a cell: its info is read once
+#pragma unroll MULTI ? EPT / GRP : 1
+                for (int gi = 0; gi < EPT / GRP; ++gi) {
+                    const int c = (tid + gi * GRP * NT) / (2 * Q);
+                    const CellInfo ci = info[c];
+                    unsigned long long run = 0ull;
+                    if constexpr (MULTI) {
+#pragma unroll
+                        for (int q = 0; q < GRP; ++q) {
+                            const int i = gi * GRP + q;
+                            element((tid + i * NT) % (2 * Q), c, ci, i, run);
+                            __builtin_amdgcn_sched_barrier(0);   // keep per-element live ranges short
+                        }
+                    } else {
+#pragma unroll 2
+                        for (int q = 0; q < GRP; ++q) {
+                            const int i = gi * GRP + q;
+                            element((tid + i * NT) % (2 * Q), c, ci, i, run);
+                        }
+#pragma unroll
+                        for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
+                        if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
+                    }
                 }
             }
             // Nyquist bin k = M of g = 0 cells: X[M] = Re Z0 - Im Z0, u real, E = 1
