@@ -59,6 +59,8 @@ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x
 // multiply by -i
 __device__ inline float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
 // forward DFTs, w = e^{-2 pi i / R}
 __device__ inline void dft2(float2* v) {
     float2 a = v[0], b = v[1];
@@ -92,14 +94,49 @@ __device__ inline void dft8(float2* v) {
     v[3] = cadd(e[3], o3);
     v[7] = csub(e[3], o3);
 }
+// 16 points as two interleaved 8-point transforms and the e^{-2 pi i k / 16} butterflies
+__device__ inline void dft16(float2* v) {
+    const float c1 = 0.92387953251128675613f;   // cos(pi / 8)
+    const float s1 = 0.38268343236508977173f;   // sin(pi / 8)
+    const float r = 0.70710678118654752440f;
+    float2 e[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+    }
+    dft8(e);
+    dft8(o);
+    float2 t[8];
+    t[0] = o[0];
+    t[1] = make_float2(c1 * o[1].x + s1 * o[1].y, c1 * o[1].y - s1 * o[1].x);
+    t[2] = make_float2(r * (o[2].x + o[2].y), r * (o[2].y - o[2].x));
+    t[3] = make_float2(s1 * o[3].x + c1 * o[3].y, s1 * o[3].y - c1 * o[3].x);
+    t[4] = cmul_mi(o[4]);
+    t[5] = make_float2(c1 * o[5].y - s1 * o[5].x, -(s1 * o[5].y + c1 * o[5].x));
+    t[6] = make_float2(r * (o[6].y - o[6].x), -r * (o[6].x + o[6].y));
+    t[7] = make_float2(s1 * o[7].y - c1 * o[7].x, -(c1 * o[7].y + s1 * o[7].x));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        v[k] = cadd(e[k], t[k]);
+        v[k + 8] = csub(e[k], t[k]);
+    }
+}
 template <int R>
 __device__ inline void dft(float2* v) {
     if constexpr (R == 2) dft2(v);
     else if constexpr (R == 4) dft4(v);
-    else dft8(v);
+    else if constexpr (R == 8) dft8(v);
+    else dft16(v);
 }
 
-constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+// radix of the Stockham pass at sub-length Ns: 8 where the remaining length
+// is a power of 8, otherwise a 16 first so that the passes stay at
+// ceil(log2(left) / 3) or fewer (Q = 8192 = 16 * 8^3, 2048 = 16 * 16 * 8)
+constexpr int pick_radix(int left) {
+    return left <= 2 ? left : left == 4 ? 4 : (left == 8 || left == 32) ? 8
+         : (ilog2c(left) % 3 == 0 ? 8 : 16);
+}
 
 // e^{2 pi i rev} from the hardware sine/cosine (argument in revolutions;
 // absolute error 1.3e-7 for |rev| <= 1.5, measured on gfx950)
@@ -179,11 +216,15 @@ __device__ inline void stockham_pass(float2* buf, const float2* __restrict__ tw,
                 w[2] = cmul(w[1], w[1]);
                 w[3] = cmul(w[2], w[1]);
             }
-            if constexpr (R == 8) {
+            if constexpr (R >= 8) {
                 w[4] = cmul(w[2], w[2]);
                 w[5] = cmul(w[4], w[1]);
                 w[6] = cmul(w[3], w[3]);
                 w[7] = cmul(w[4], w[3]);
+            }
+            if constexpr (R == 16) {
+#pragma unroll
+                for (int r = 8; r < 16; ++r) w[r] = cmul(w[r / 2], w[r - r / 2]);   // depth <= 4
             }
 #pragma unroll
             for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r]);
@@ -200,7 +241,7 @@ template <int Q, int NT, int NPTS, int Ns>
 __device__ inline void fft_passes(float2* buf, const float2* __restrict__ tw, int lgTWN) {
     if constexpr (Ns < Q) {
         constexpr int left = Q / Ns;
-        constexpr int R = left >= 8 ? 8 : left;
+        constexpr int R = pick_radix(left);
         stockham_pass<Q, NT, NPTS, Ns, R>(buf, tw, lgTWN);
         fft_passes<Q, NT, NPTS, Ns * R>(buf, tw, lgTWN);
     }
