@@ -523,22 +523,9 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             // post: split the packed spectrum, phase-correct, product per bin.
             // element(rem, c, ci, i, run): bin `rem` (of 2Q) of cell c; i indexes
             // the multichannel accumulators.
-            auto element = [&](const int rem, const int c, const CellInfo& ci, const int i,
-                               unsigned long long& run) {
-                const int half = rem / Q, k1 = rem % Q;
-                const int k2 = half ? ci.kb : ci.ka;
-                const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
-                if (!(ci.valid && k2 >= 0 && k < F)) return;
-                const int slot = 2 * c + half;
-                const float2 Zk = buf[lpad(slot * Q + k1)];
-                float2 Zp;
-                if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
-                else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
-                const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
-                const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
-                const float2 wv = cis_rev(-static_cast<float>(k) * invN);
-                const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
-                const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
+            // product of bin k from its spectrum value X = Xr + i Xi
+            auto from_x = [&](const int k, const float Xr, const float Xi, const CellInfo& ci, const int i,
+                              unsigned long long& run) {
                 const bool high = 4 * k > Nfft;
                 const int j = high ? M - k : k;
                 const float num = Xr * Xr + Xi * Xi;
@@ -578,6 +565,61 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     acc0[i] += pe;
                 }
             };
+            auto element = [&](const int rem, const int c, const CellInfo& ci, const int i,
+                               unsigned long long& run) {
+                const int half = rem / Q, k1 = rem % Q;
+                const int k2 = half ? ci.kb : ci.ka;
+                const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
+                if (!(ci.valid && k2 >= 0 && k < F)) return;
+                const int slot = 2 * c + half;
+                const float2 Zk = buf[lpad(slot * Q + k1)];
+                float2 Zp;
+                if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
+                else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
+                const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
+                const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
+                const float2 wv = cis_rev(-static_cast<float>(k) * invN);
+                const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
+                const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
+                from_x(k, Xr, Xi, ci, i, run);
+            };
+            // Two bins from one pair of LDS values (single-channel units,
+            // Q >= NT): bin k and its mirror M - k share A = Z[k] + conj Z[M-k],
+            // B = Z[k] - conj Z[M-k] and w^k: X[k] = (A - i w^k B) / 2,
+            // X[M-k] = conj(A + i w^k B) / 2.  Pair u < Q of cell c is (slot a, u)
+            // with (slot b, Q-1-u) for g > 0; for g = 0, (a, u) with (a, Q-u) for
+            // u < Q/2 and (b, u') with (b, Q-1-u'), u' = u - Q/2; u = 0 also takes
+            // the self-mirrored (a, Q/2).  The Nyquist bin keeps its own path.
+            auto pair = [&](const int u, const int c, const CellInfo& ci, unsigned long long& run) {
+                if (!ci.valid) return;
+                int sa, ka1, sb, kb1, k2;
+                if (ci.ka != 0) {
+                    sa = 2 * c; ka1 = u; sb = 2 * c + 1; kb1 = Q - 1 - u; k2 = ci.ka;
+                } else if (u < Q / 2) {
+                    sa = sb = 2 * c; ka1 = u; kb1 = (Q - u) & (Q - 1); k2 = 0;
+                } else {
+                    if (ci.kb < 0) return;
+                    sa = sb = 2 * c + 1; ka1 = u - Q / 2; kb1 = Q - 1 - ka1; k2 = ci.kb;
+                }
+                const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(ka1))) + k2;
+                const int km = M - k;
+                const float2 Zk = buf[lpad(sa * Q + ka1)];
+                const float2 Zp = buf[lpad(sb * Q + kb1)];
+                const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
+                const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
+                const float2 wv = cis_rev(-static_cast<float>(k) * invN);
+                const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
+                if (k < F) from_x(k, 0.5f * (Ar + pi), 0.5f * (Ai - pr), ci, 0, run);
+                if (km < F && km != M) from_x(km, 0.5f * (Ar - pi), -0.5f * (Ai + pr), ci, 0, run);
+                if (ci.ka == 0 && u == 0 && M / 2 < F) {
+                    // (a, Q/2): bin M/2, its own mirror (Z[k] = Z[M-k])
+                    const float2 Zh = buf[lpad(2 * c * Q + Q / 2)];
+                    const float2 wh = cis_rev(-static_cast<float>(M / 2) * invN);
+                    const float hBi = Zh.y + Zh.y;                 // B = (0, 2 Im Zh), A = (2 Re Zh, 0)
+                    const float hpr = -wh.y * hBi, hpi = wh.x * hBi;
+                    from_x(M / 2, 0.5f * (Zh.x + Zh.x + hpi), -0.5f * hpr, ci, 0, run);
+                }
+            };
             constexpr bool CONTIG = !MULTI && Q >= 8 && Q <= 256;
             if constexpr (CONTIG) {
                 // Small sub-transforms: a thread's EPT elements are consecutive
@@ -608,10 +650,11 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                             __builtin_amdgcn_sched_barrier(0);   // keep per-element live ranges short
                         }
                     } else {
+                        if constexpr (GRP >= 2) {
 #pragma unroll 2
-                        for (int q = 0; q < GRP; ++q) {
-                            const int i = gi * GRP + q;
-                            element((tid + i * NT) % (2 * Q), c, ci, i, run);
+                            for (int q = 0; q < GRP / 2; ++q) pair(tid + q * NT, c, ci, run);
+                        } else {
+                            element((tid + gi * NT) % (2 * Q), c, ci, gi, run);
                         }
 #pragma unroll
                         for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
