@@ -48,3 +48,27 @@ def test_host_side_argument_checks():
     assert ei.value.status == 4
     scales, entries = E.dictionary(128.0, 1280, 0.01)
     assert entries == 5008073 and len(scales) == 34
+
+
+def test_plan_create_rejects_bad_items_before_any_device_call():
+    """empi_plan_create validates every item (the settings checks of
+    src/empi.cpp:59-112) and the kernels' size limit before touching a
+    device, so these run on a host without a GPU."""
+    bad = [
+        E.ItemSpec(0, 1280, 128.0, E.SMP, 10, 99.0, 0.01),      # no channels
+        E.ItemSpec(1, 1, 128.0, E.SMP, 10, 99.0, 0.01),         # one sample
+        E.ItemSpec(1, 1280, 0.0, E.SMP, 10, 99.0, 0.01),        # fs
+        E.ItemSpec(1, 1280, 128.0, E.SMP, 0, 99.0, 0.01),       # iterationMax
+        E.ItemSpec(1, 1280, 128.0, 7, 10, 99.0, 0.01),          # variant
+        E.ItemSpec(1, 1280, 128.0, E.SMP, 10, 101.0, 0.01),     # energyPercent > 100
+        E.ItemSpec(1, 1280, 128.0, E.SMP, 10, 0.0, 0.01),       # residualEnergy 1
+        E.ItemSpec(1, 1280, 128.0, E.SMP, 10, 99.0, 1.5),       # energyError outside (0, 1)
+    ]
+    for spec in bad:
+        with pytest.raises(E.EmpiError) as ei:
+            E.Plan([spec])
+        assert ei.value.status == 4, spec
+    # a window beyond 16,384 samples needs a sub-transform longer than 8192
+    with pytest.raises(E.EmpiError) as ei:
+        E.Plan([E.ItemSpec(1, 40000, 256.0, E.SMP, 10, 99.0, 0.01)])
+    assert ei.value.status == 8
