@@ -185,7 +185,7 @@ struct Lds {
 
 // One Stockham autosort pass of radix R at sub-length Ns over all slots.
 template <int Q, int NT, int NPTS, int Ns, int R>
-__device__ inline void stockham_pass(float2* buf, const float2* __restrict__ tw, int lgTWN) {
+__device__ inline void stockham_pass(float2* buf) {
     constexpr int BPS = Q / R;
     constexpr int NB = NPTS / R;
     constexpr int PER = NB / NT;
@@ -239,12 +239,12 @@ __device__ inline void stockham_pass(float2* buf, const float2* __restrict__ tw,
 }
 
 template <int Q, int NT, int NPTS, int Ns>
-__device__ inline void fft_passes(float2* buf, const float2* __restrict__ tw, int lgTWN) {
+__device__ inline void fft_passes(float2* buf) {
     if constexpr (Ns < Q) {
         constexpr int left = Q / Ns;
         constexpr int R = pick_radix(left);
-        stockham_pass<Q, NT, NPTS, Ns, R>(buf, tw, lgTWN);
-        fft_passes<Q, NT, NPTS, Ns * R>(buf, tw, lgTWN);
+        stockham_pass<Q, NT, NPTS, Ns, R>(buf);
+        fft_passes<Q, NT, NPTS, Ns * R>(buf);
     }
 }
 
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     const ClassDev* __restrict__ classes, int cls, const ScaleDev* __restrict__ scales,
     const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
     const float* __restrict__ residual, unsigned long long* __restrict__ cellbest,
-    unsigned long long* __restrict__ tilebest, const float2* __restrict__ tw, int lgTWN, int full,
+    unsigned long long* __restrict__ tilebest,
     unsigned long long* __restrict__ counter) {
     using G_ = Geo<Q>;
     constexpr int NT = G_::NT, NPTS = G_::NPTS, NCELL = G_::NCELL, EPT = G_::EPT;
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 }
             }
             __syncthreads();
-            fft_passes<Q, NT, NPTS, 1>(buf, tw, lgTWN);
+            fft_passes<Q, NT, NPTS, 1>(buf);
 
             // post: split the packed spectrum, phase-correct, product per bin.
             // element(rem, c, ci, i, run): bin `rem` (of 2Q) of cell c; i indexes
@@ -1711,16 +1711,15 @@ constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
 
 typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
                                const UnitDev* units, const UnitState* state, const float* residual,
-                               unsigned long long* cellbest, unsigned long long* tilebest, const float2* tw,
-                               int lgTWN, int full, unsigned long long* counter);
+                               unsigned long long* cellbest, unsigned long long* tilebest,
+                               unsigned long long* counter);
 
 template <int Q, int MODE, int W>
 void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
                     const UnitDev* units, const UnitState* state, const float* residual,
-                    unsigned long long* cellbest, unsigned long long* tilebest, const float2* tw, int lgTWN,
-                    int full, unsigned long long* counter) {
+                    unsigned long long* cellbest, unsigned long long* tilebest, unsigned long long* counter) {
     hipLaunchKernelGGL((k_columns<Q, MODE, W>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
-                       state, residual, cellbest, tilebest, tw, lgTWN, full, counter);
+                       state, residual, cellbest, tilebest, counter);
 }
 
 // persistent grid: enough workgroups to fill 256 CUs a few times over (twice
@@ -1781,7 +1780,6 @@ struct empi_plan {
     ClassJobs classes[3][kNumClasses];   // [mode][class]
     int tune[3][kNumClasses];            // column-kernel variant per (mode, class): wave + 3 * grid variant
     int maxIter = 0;
-    int twN = 2, lgTWN = 1;
     long long total_in = 0;
 
     DevBuf<ScaleDev> d_scales;
@@ -1795,7 +1793,6 @@ struct empi_plan {
     DevBuf<ClassDev> d_classes;
     std::vector<UnitScale> uscales_h;
     int nclass_dev = 0;
-    DevBuf<float2> d_tw;
     DevBuf<float2> d_etab;
     DevBuf<empi_atom> d_atoms;
     DevBuf<double> d_energies;
@@ -1997,7 +1994,6 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     p->total_in = in_off;
 
     // scales of every dictionary
-    int maxNfft = 2;
     std::vector<float2> etab_h;
     for (size_t di = 0; di < p->dicts.size(); ++di) {
         const Dictionary& d = p->dicts[di];
@@ -2044,12 +2040,9 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
                 s.csnorm = static_cast<float>(proposal_bound_factor(sc.s, d.fs, sc.Nfft));
                 s.global = 6.0 * sfs >= 0.25 * d.N ? 1 : 0;
             }
-            maxNfft = std::max(maxNfft, sc.Nfft);
             p->scales_h.push_back(s);
         }
     }
-    p->twN = maxNfft;
-    p->lgTWN = lg2(maxNfft);
 
     // region tables: time indices of each (scale, 256-sample block); column t
     // sits at sample t (N-1)/(T-1) and belongs to block floor(of that / 256)
@@ -2121,7 +2114,6 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_uscales.alloc(p->uscales_h.size())) != EMPI_OK) return st;
     if ((st = p->d_etab.alloc(etab_h.size())) != EMPI_OK) return st;
     for (ScaleDev& sd : p->scales_h) sd.etab = p->d_etab.p + reinterpret_cast<uintptr_t>(sd.etab);
-    if ((st = p->d_tw.alloc(static_cast<size_t>(p->twN))) != EMPI_OK) return st;
     if ((st = p->d_atoms.alloc(static_cast<size_t>(atom_off))) != EMPI_OK) return st;
     if ((st = p->d_energies.alloc(static_cast<size_t>(energy_off))) != EMPI_OK) return st;
     if ((st = p->d_fitv.alloc(p->units.size() * static_cast<size_t>(max_c))) != EMPI_OK) return st;
@@ -2133,15 +2125,9 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
         u.energies = p->d_energies.p + reinterpret_cast<long long>(u.energies);
         u.fitv = p->d_fitv.p + ui * static_cast<size_t>(max_c);
     }
-    std::vector<float2> tw(static_cast<size_t>(p->twN));
-    for (int j = 0; j < p->twN; ++j) {
-        const double a = -2.0 * M_PI * static_cast<double>(j) / static_cast<double>(p->twN);
-        tw[j] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
-    }
     hipStream_t s0 = nullptr;
     if ((st = p->d_scales.upload(p->scales_h.data(), p->scales_h.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_units.upload(p->units.data(), p->units.size(), s0)) != EMPI_OK) return st;
-    if ((st = p->d_tw.upload(tw.data(), tw.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_etab.upload(etab_h.data(), etab_h.size(), s0)) != EMPI_OK) return st;
     std::vector<ClassDev> cds;
     for (int mode = 0; mode < 3; ++mode) {
@@ -2181,7 +2167,7 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
 }
 
 // column kernels of every class over the current work lists
-empi_status launch_class_columns(empi_plan* p, hipStream_t s, int full) {
+empi_status launch_class_columns(empi_plan* p, hipStream_t s) {
     for (int mc = 0; mc < 3 * kNumClasses; ++mc) {
         const int mode = mc / kNumClasses, c = mc % kNumClasses;
         ClassJobs& cj = p->classes[mode][c];
@@ -2200,7 +2186,7 @@ empi_status launch_class_columns(empi_plan* p, hipStream_t s, int full) {
         const int choice = p->tune[mode][c];
         column_launcher(mode, c, choice)(column_grid(1 << (c + kMinLgQ), cj.tiles, choice / kWaveVariants), s,
                                          p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p, p->d_state.p,
-                                         p->d_res32.p, p->d_cells.p, p->d_tiles.p, p->d_tw.p, p->lgTWN, full,
+                                         p->d_res32.p, p->d_cells.p, p->d_tiles.p,
                                          p->d_counter.p);
         if (evp) EMPI_HIP(hipEventRecord(evp->second, s));
     }
@@ -2212,7 +2198,7 @@ empi_status launch_all_columns(empi_plan* p, hipStream_t s, int full) {
     if (p->nclass_dev == 0) return EMPI_OK;
     hipLaunchKernelGGL(k_plan, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p, p->d_state.p,
                        p->d_scales.p, full);
-    return launch_class_columns(p, s, full);
+    return launch_class_columns(p, s);
 }
 
 // ------------------------------------------------------------ proposal host side
@@ -2269,7 +2255,7 @@ empi_status proposal_pass(empi_plan* p, hipStream_t s, unsigned long long* stat)
                        p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_regtab.p, p->d_regions.p, stat);
     hipLaunchKernelGGL(k_scan_items, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p);
     EMPI_HIP(hipGetLastError());
-    empi_status st = launch_class_columns(p, s, 0);
+    empi_status st = launch_class_columns(p, s);
     if (st != EMPI_OK) return st;
     hipLaunchKernelGGL(k_regions_reduce, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
                        p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_regions.p, 0);
@@ -2386,7 +2372,7 @@ empi_status plan_autotune(empi_plan* p, const float* signal, int on_device, int 
                     column_launcher(mode, c, choice)(column_grid(1 << (c + kMinLgQ), cj.tiles, choice / kWaveVariants),
                                                      s, p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p,
                                                      p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p,
-                                                     p->d_tw.p, p->lgTWN, 1, nullptr);
+                                                     nullptr);
                     float ms = 0.0f;
                     if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
