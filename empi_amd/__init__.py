@@ -33,7 +33,7 @@ EXPORTS = [
     "empi_plan_entries_evaluated", "empi_plan_set_profiling", "empi_plan_kernel_classes",
     "empi_plan_kernel_stats", "empi_plan_destroy", "empi_strerror", "empi_version",
     "empi_mlp_create", "empi_mlp_destroy", "empi_mlp_param_count", "empi_mlp_get_params",
-    "empi_mlp_set_params", "empi_mlp_forward", "empi_mlp_gradients", "empi_mlp_train",
+    "empi_mlp_set_params", "empi_mlp_forward", "empi_mlp_forward16", "empi_mlp_gradients", "empi_mlp_train",
     "empi_plan_tuning_size", "empi_plan_autotune", "empi_plan_get_tuning", "empi_plan_set_tuning",
     "empi_proposal_feature_dim", "empi_proposal_bound_factor", "empi_plan_set_proposal", "empi_plan_proposal_stats",
     "empi_plan_set_training_capacity", "empi_plan_training_rows",
@@ -116,6 +116,7 @@ def lib():
         L.empi_mlp_get_params.argtypes = [vp, fp]
         L.empi_mlp_set_params.argtypes = [vp, fp]
         L.empi_mlp_forward.argtypes = [vp, vp, C.c_int, vp, C.c_int]
+        L.empi_mlp_forward16.argtypes = [vp, vp, C.c_int, vp, C.c_int]
         L.empi_mlp_gradients.argtypes = [vp, fp, fp, C.c_int, fp, fp]
         L.empi_mlp_train.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, fp]
         L.empi_plan_tuning_size.restype = C.c_int
@@ -391,6 +392,14 @@ class MLP:
         y = np.zeros(X.shape[0], dtype=np.float32)
         _check(lib().empi_mlp_forward(self._h, X.ctypes.data_as(C.c_void_p), X.shape[0],
                                       y.ctypes.data_as(C.c_void_p), 0), "empi_mlp_forward")
+        return y
+
+    def forward16(self, X):
+        """The proposal's inference pass (FP16 inputs and hidden layer, matrix cores)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        _check(lib().empi_mlp_forward16(self._h, X.ctypes.data_as(C.c_void_p), X.shape[0],
+                                        y.ctypes.data_as(C.c_void_p), 0), "empi_mlp_forward16")
         return y
 
     def gradients(self, X, t):

@@ -95,6 +95,97 @@ __global__ __launch_bounds__(NT) void k_mlp_forward(const float* __restrict__ X,
     }
 }
 
+// Inference on the matrix cores: inputs and the first hidden layer rounded
+// to FP16, FP16 weights, FP32 accumulation (v_mfma_f32_32x32x8f16).  One
+// workgroup takes 32 rows; wave w computes hidden units [64 w, 64 w + 64) as
+// two 32-column tiles.  Operand layout of the instruction (lane l, k < 4):
+// A[l % 32][4 (l / 32) + k], B[4 (l / 32) + k][l % 32]; result j < 16 is
+// D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32].
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+constexpr int kMfmaRows = 32;
+constexpr int kMaxDin = 256;
+
+__global__ __launch_bounds__(NT) void k_mlp_forward_mfma(const float* __restrict__ X, int R,
+                                                         const int* __restrict__ Rdev, int din,
+                                                         const __half* __restrict__ W1, const float* __restrict__ b1,
+                                                         const __half* __restrict__ W2, const float* __restrict__ b2,
+                                                         const __half* __restrict__ w3, const float* __restrict__ b3,
+                                                         float* __restrict__ y) {
+    __shared__ _Float16 xs[kMfmaRows][kMaxDin + 8];
+    __shared__ _Float16 h1[kMfmaRows][H + 8];
+    __shared__ float red[NT / 64][kMfmaRows];
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * kMfmaRows;
+    if (Rdev) R = min(R, *Rdev);
+    if (r0 >= R) return;   // uniform over the workgroup
+    const int kp = (din + 7) / 8 * 8;
+    for (int i = tid; i < kMfmaRows * kp; i += NT) {
+        const int r = i / kp, c = i % kp;
+        xs[r][c] = static_cast<_Float16>(r0 + r < R && c < din ? X[static_cast<long long>(r0 + r) * din + c] : 0.0f);
+    }
+    __syncthreads();
+    const int wave = tid >> 6, l = tid & 63;
+    const int lr = l % 32, lk = 4 * (l / 32);
+    const __half_raw* W1r = reinterpret_cast<const __half_raw*>(W1);
+    const __half_raw* W2r = reinterpret_cast<const __half_raw*>(W2);
+    auto hw = [](const __half_raw* w, long long i) {
+        _Float16 v;
+        __builtin_memcpy(&v, &w[i].data, sizeof v);
+        return v;
+    };
+    // layer 1
+    for (int nt = 0; nt < 2; ++nt) {
+        const int col = 64 * wave + 32 * nt + lr;
+        float16_t acc;
+        for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+        for (int k0 = 0; k0 < kp; k0 += 8) {
+            half4_t a, b;
+            for (int k = 0; k < 4; ++k) {
+                a[k] = xs[lr][k0 + lk + k];
+                const int kk = k0 + lk + k;
+                b[k] = kk < din ? hw(W1r, static_cast<long long>(col) * din + kk) : static_cast<_Float16>(0.0f);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
+        }
+        const float bias = b1[col];
+        for (int j = 0; j < 16; ++j) h1[8 * (j / 4) + lk + (j % 4)][col] = static_cast<_Float16>(fmaxf(acc[j] + bias, 0.0f));
+    }
+    __syncthreads();
+    // layer 2 and the output dot product
+    float part[16];
+    for (int j = 0; j < 16; ++j) part[j] = 0.0f;
+    for (int nt = 0; nt < 2; ++nt) {
+        const int col = 64 * wave + 32 * nt + lr;
+        float16_t acc;
+        for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+        for (int k0 = 0; k0 < H; k0 += 8) {
+            half4_t a, b;
+            for (int k = 0; k < 4; ++k) {
+                a[k] = h1[lr][k0 + lk + k];
+                b[k] = hw(W2r, static_cast<long long>(col) * H + k0 + lk + k);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
+        }
+        const float bias = b2[col], wo = __half2float(w3[col]);
+        for (int j = 0; j < 16; ++j) part[j] = fmaf(fmaxf(acc[j] + bias, 0.0f), wo, part[j]);
+    }
+    // sum over the 32 columns held by the lanes of each half-wave
+    for (int j = 0; j < 16; ++j) {
+        float v = part[j];
+        for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+        part[j] = v;
+    }
+    if (lr == 0)
+        for (int j = 0; j < 16; ++j) red[wave][8 * (j / 4) + lk + (j % 4)] = part[j];
+    __syncthreads();
+    if (tid < kMfmaRows && r0 + tid < R) {
+        float v = b3[0];
+        for (int w = 0; w < NT / 64; ++w) v += red[w][tid];
+        y[r0 + tid] = v;
+    }
+}
+
 // dy = 2 (y - t) / R and the summed squared error
 __global__ void k_mlp_loss(const float* __restrict__ y, const float* __restrict__ t, int R, float* __restrict__ dy,
                            float* __restrict__ loss) {
@@ -278,6 +369,17 @@ empi_status forward(empi_mlp* m, const float* X, int R, float* y, bool keep) {
     return EMPI_OK;
 }
 
+// the matrix-core inference pass (FP16 inputs and hidden layer)
+empi_status forward16(empi_mlp* m, const float* X, int R, const int* Rdev, float* y, hipStream_t s) {
+    if (m->din > kMaxDin) return EMPI_UNSUPPORTED;
+    const int blocks = (R + kMfmaRows - 1) / kMfmaRows;
+    hipLaunchKernelGGL(k_mlp_forward_mfma, dim3(blocks), dim3(NT), 0, s, X, R, Rdev, m->din, m->wh.p + m->oW1(),
+                       m->w.p + m->ob1(), m->wh.p + m->oW2(), m->w.p + m->ob2(), m->wh.p + m->ow3(),
+                       m->w.p + m->ob3(), y);
+    MLP_HIP(hipGetLastError());
+    return EMPI_OK;
+}
+
 empi_status gradients(empi_mlp* m, const float* X, const float* t, int R, float* loss_out) {
     empi_status st = ensure_rows(m, static_cast<size_t>(R));
     if (st) return st;
@@ -384,6 +486,24 @@ empi_status empi_mlp_forward(empi_mlp* m, const float* X, int rows, float* y, in
     return EMPI_OK;
 }
 
+empi_status empi_mlp_forward16(empi_mlp* m, const float* X, int rows, float* y, int on_device) {
+    if (!m || !X || !y || rows < 0) return EMPI_INVALID_ARGUMENT;
+    if (rows == 0) return EMPI_OK;
+    empi_status st;
+    if (on_device) {
+        if ((st = forward16(m, X, rows, nullptr, y, m->stream))) return st;
+        MLP_HIP(hipStreamSynchronize(m->stream));
+        return EMPI_OK;
+    }
+    Dev<float> dx, dyv;
+    if ((st = dx.alloc(static_cast<size_t>(rows) * m->din)) || (st = dyv.alloc(rows))) return st;
+    MLP_HIP(hipMemcpy(dx.p, X, sizeof(float) * rows * m->din, hipMemcpyHostToDevice));
+    if ((st = forward16(m, dx.p, rows, nullptr, dyv.p, m->stream))) return st;
+    MLP_HIP(hipStreamSynchronize(m->stream));
+    MLP_HIP(hipMemcpy(y, dyv.p, sizeof(float) * rows, hipMemcpyDeviceToHost));
+    return EMPI_OK;
+}
+
 empi_status empi_mlp_gradients(empi_mlp* m, const float* X, const float* target, int rows, float* grads,
                                float* loss) {
     if (!m || !X || !target || rows <= 0) return EMPI_INVALID_ARGUMENT;
@@ -450,13 +570,7 @@ empi_status empi_mlp_forward_counted(empi_mlp* m, const float* X, const int* row
                                      hipStream_t s) {
     if (!m || !X || !rows_dev || !y || max_rows < 0) return EMPI_INVALID_ARGUMENT;
     if (max_rows == 0) return EMPI_OK;
-    const int blocks = (max_rows + ROWS - 1) / ROWS;
-    const size_t shm = sizeof(float) * ROWS * (m->din + H);
-    hipLaunchKernelGGL(k_mlp_forward, dim3(blocks), dim3(NT), shm, s, X, max_rows, rows_dev, m->din,
-                       m->wh.p + m->oW1(), m->w.p + m->ob1(), m->wh.p + m->oW2(), m->w.p + m->ob2(),
-                       m->wh.p + m->ow3(), m->w.p + m->ob3(), y, nullptr, nullptr);
-    MLP_HIP(hipGetLastError());
-    return EMPI_OK;
+    return forward16(m, X, max_rows, rows_dev, y, s);
 }
 
 int empi_mlp_input_dim(const empi_mlp* m) { return m ? m->din : 0; }

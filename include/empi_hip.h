@@ -186,6 +186,9 @@ empi_status empi_mlp_get_params(const empi_mlp* mlp, float* out);
 empi_status empi_mlp_set_params(empi_mlp* mlp, const float* in);
 /* y[rows] from X[rows][input_dim]; both on the device when on_device != 0. */
 empi_status empi_mlp_forward(empi_mlp* mlp, const float* X, int rows, float* y, int on_device);
+/* The inference pass the proposal runs: inputs and the first hidden layer
+ * rounded to FP16, products on the matrix cores with FP32 accumulation. */
+empi_status empi_mlp_forward16(empi_mlp* mlp, const float* X, int rows, float* y, int on_device);
 /* Gradients of the mean squared error over one batch (host arrays), in the
  * parameter layout; *loss receives the mean squared error. */
 empi_status empi_mlp_gradients(empi_mlp* mlp, const float* X, const float* target, int rows, float* grads,

@@ -27,9 +27,11 @@ def half(a):
     return np.asarray(a, dtype=np.float32).astype(np.float16).astype(np.float64)
 
 
-def forward(p, X, rounded=True):
+def forward(p, X, rounded=True, act16=False):
     """rounded: weights through FP16 and biases through FP32, as the device
-    reads them; False gives the exact FP64 network (finite-difference checks)."""
+    reads them; False gives the exact FP64 network (finite-difference checks).
+    act16: inputs and first hidden layer also through FP16, as the
+    matrix-core inference pass reads them."""
     din = X.shape[1]
     W1, b1, W2, b2, w3, b3 = unpack(p, din)
     if rounded:
@@ -37,7 +39,11 @@ def forward(p, X, rounded=True):
         b1 = b1.astype(np.float32).astype(np.float64)
         b2 = b2.astype(np.float32).astype(np.float64)
         b3 = np.float64(np.float32(b3))
+    if act16:
+        X = half(X)
     h1 = np.maximum(X @ W1.T + b1, 0.0)
+    if act16:
+        h1 = half(h1)
     h2 = np.maximum(h1 @ W2.T + b2, 0.0)
     return h2 @ w3 + b3, h1, h2
 

@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "empi_hip.h")
 def declared():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(empi_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(empi_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_expected_entry_points():
@@ -30,7 +30,7 @@ def test_library_exports_all_symbols():
         assert hasattr(lib, name), name
     out = subprocess.run(["nm", "-D", "--defined-only", E.LIB_PATH], capture_output=True, text=True,
                          check=True).stdout
-    exported = set(re.findall(r"\sT\s(empi_[a-z_]+)$", out, flags=re.M))
+    exported = set(re.findall(r"\sT\s(empi_[a-z0-9_]+)$", out, flags=re.M))
     assert set(declared()) <= exported
 
 

@@ -46,3 +46,30 @@ def test_training_reduces_loss():
     net.train(X, t, epochs=20, batch=1024, lr=2e-3)
     after = float(np.mean((net.forward(X[:2000]) - t[:2000]) ** 2))
     assert after < 0.1 * before, (before, after)
+
+
+@pytest.mark.parametrize("din,rows", [(80, 200), (13, 45)])
+def test_matrix_core_inference_matches_oracle(din, rows):
+    """The proposal's inference pass (FP16 inputs and first hidden layer on
+    v_mfma_f32_32x32x8f16) against the FP64 oracle with the same roundings,
+    on the committed network and on an odd width / row count."""
+    print(f"\n[gpu case] MLP matrix-core forward din={din} rows={rows}", flush=True)
+    from empi_amd import proposal
+    rng = np.random.default_rng(5 + din)
+    if din == 80 and proposal.load() is not None:
+        net = proposal.load()
+        X = rng.normal(-8, 4, (rows, din)).astype(np.float32)
+    else:
+        net = E.MLP(din, seed=9)
+        p = net.params()
+        p[256 * din:256 * din + 256] = rng.normal(0, 0.1, 256)
+        net.set_params(p)
+        X = rng.normal(0, 1, (rows, din)).astype(np.float32)
+    p = net.params()
+    y = net.forward16(X)
+    y_ref, _, _ = mo.forward(p, X.astype(np.float64), act16=True)
+    scale = max(1.0, np.max(np.abs(y_ref)))
+    assert np.max(np.abs(y - y_ref)) <= 1e-4 * scale, np.max(np.abs(y - y_ref))
+    # the FP32-activation pass differs by the roundings only
+    y32 = net.forward(X)
+    assert np.max(np.abs(y - y32)) <= 2e-2 * scale
