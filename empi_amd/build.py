@@ -19,7 +19,11 @@ HIP_FLAGS = [
     "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "csrc"),
 ]
 
-ENGINE_SOURCES = ["csrc/engine.hip", "csrc/mlp.hip", "csrc/dictionary.cpp"]
+# the column kernels are split by class range so that the translation units
+# compile in parallel
+ENGINE_SOURCES = ["csrc/columns_large.hip", "csrc/columns_mid.hip", "csrc/columns_small.hip", "csrc/engine.hip",
+                  "csrc/mlp.hip", "csrc/dictionary.cpp"]
+OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libempi_hip.so")
 
 HOST_SOURCES = ["host/empi_main.cpp", "host/workspace.cpp", "host/config.cpp", "host/book.cpp",
@@ -34,10 +38,19 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_engine():
-    srcs = [os.path.join(HERE, s) for s in ENGINE_SOURCES]
+def build_engine(jobs=None):
+    """Compiles every engine source to an object (in parallel) and links
+    libempi_hip.so."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = [os.path.join(OBJ_DIR, os.path.basename(s) + ".o") for s in ENGINE_SOURCES]
+    cmds = [[HIPCC] + HIP_FLAGS + ["-c", "-o", o, os.path.join(HERE, s)] for s, o in zip(ENGINE_SOURCES, objs)]
+    jobs = jobs or min(len(cmds), max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs)
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     return LIB
 

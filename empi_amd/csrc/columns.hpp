@@ -1,0 +1,56 @@
+// FFT-size classes of the column kernels (Q = 2 .. 8192 complex points per
+// slot), their workgroup geometry, modes and tuning variants, and the launch
+// table the host plan dispatches through.  The kernels are compiled in three
+// translation units by class range; each exports one launcher lookup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace empi {
+
+constexpr int kMaxQ = 8192;
+constexpr int kMinLgQ = 1;    // Q = 2
+constexpr int kMaxLgQ = 13;   // Q = 8192
+constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
+
+// Workgroup geometry of class Q: NSL slots of Q complex points in LDS,
+// 16 points per thread.
+template <int Q>
+struct Geo {
+    static constexpr int NSL = Q >= 2048 ? 2 : (4096 / Q > 512 ? 512 : 4096 / Q);
+    static constexpr int NPTS = NSL * Q;
+    static constexpr int EPT = 16;
+    static constexpr int NT = NPTS / EPT;
+    static constexpr int NCELL = NSL / 2;
+};
+
+// column kernel modes: one channel; channel sum of products (MMP3 and
+// multichannel workspaces); MMP1 constraint (three accumulators per bin)
+enum { kModeSingle = 0, kModeSum = 1, kModeMmp1 = 2 };
+// waves per SIMD the register allocator must allow in the multichannel modes
+template <int Q, int MODE>
+constexpr int column_waves() { return MODE == kModeSum ? 2 : 1; }
+
+// Tuning variants of the single-channel column kernels: waves per SIMD the
+// register allocator must allow (4, 3, 2), times two persistent grid sizes.
+// Q = 8192 fills a CU's LDS with one workgroup: always 2 waves.
+constexpr int kWaveVariants = 3;
+constexpr int kGridVariants = 2;
+constexpr int kTuneChoices = kWaveVariants * kGridVariants;
+template <int Q, int V>
+constexpr int variant_waves() { return Q >= 8192 ? 4 : (V == 0 ? 4 : V == 1 ? 3 : 2); }
+
+typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
+                               const UnitDev* units, const UnitState* state, const float* residual,
+                               unsigned long long* cellbest, unsigned long long* tilebest,
+                               unsigned long long* counter);
+
+// launcher of class lgQ (log2 Q) in mode `mode`; `variant` selects the wave
+// variant of the single-channel kernels.  Q = 2..64 / 128..1024 / 2048..8192.
+ColumnLauncher column_launcher_small(int lgQ, int mode, int variant);
+ColumnLauncher column_launcher_mid(int lgQ, int mode, int variant);
+ColumnLauncher column_launcher_large(int lgQ, int mode, int variant);
+
+}  // namespace empi

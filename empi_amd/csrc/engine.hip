@@ -1,13 +1,13 @@
-// MI355X (gfx950) Matching Pursuit engine: kernels, batched plan and C ABI.
+// MI355X (gfx950) Matching Pursuit engine: unit and region kernels, batched
+// plan and C ABI.
 //
 // Hot path (one MP iteration of every unit of a batch):
 //   k_select    one workgroup per unit: argmax over the unit's cell bests,
 //               FP64 refit of the chosen column, atom parameters, FP64
 //               residual update and residual energy / stop test.
-//   k_columns<Q> for every FFT-size class Q: re-evaluates the map cells the
-//               last atom touched -- window, Gaussian, packed real FFT in LDS
-//               (Stockham radix 8/4/2), phase twiddle, product, per-cell max.
-//               The complex map is never stored: a cell best is 8 bytes.
+//   k_plan      per FFT-size class: the work list of tiles the atom touched.
+//   k_columns<Q> (column_kernel.hpp, compiled in columns_*.hip) for every
+//               class: re-evaluates the touched map cells and their bests.
 // References (paths in the reference tree): gabor.cpp:75-133,143-209,
 // 279-354; classes.cpp:26-87; mmp.cpp:9-45.
 #include <hip/hip_runtime.h>
@@ -25,6 +25,8 @@
 #include <tuple>
 #include <vector>
 
+#include "columns.hpp"
+#include "device_math.hpp"
 #include "dictionary.hpp"
 #include "empi_hip.h"
 #include "kernels.hpp"
@@ -44,251 +46,7 @@ using namespace empi;
 
 namespace {
 
-constexpr int kMaxQ = 8192;
-constexpr int kMinLgQ = 1;    // Q = 2
-constexpr int kMaxLgQ = 13;   // Q = 8192
 constexpr int kSelectThreads = 256;
-
-// ------------------------------------------------------------ device math
-
-__device__ inline float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-// multiply by -i
-__device__ inline float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
-
-constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
-
-// forward DFTs, w = e^{-2 pi i / R}
-__device__ inline void dft2(float2* v) {
-    float2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-}
-__device__ inline void dft4(float2* v) {
-    float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    float2 s13 = cadd(v[1], v[3]), d13 = cmul_mi(csub(v[1], v[3]));
-    v[0] = cadd(s02, s13);
-    v[2] = csub(s02, s13);
-    v[1] = cadd(d02, d13);
-    v[3] = csub(d02, d13);
-}
-__device__ inline void dft8(float2* v) {
-    const float r = 0.70710678118654752440f;
-    float2 e[4] = {v[0], v[2], v[4], v[6]};
-    float2 o[4] = {v[1], v[3], v[5], v[7]};
-    dft4(e);
-    dft4(o);
-    // o_k *= e^{-i pi k / 4}
-    float2 o1 = make_float2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
-    float2 o2 = cmul_mi(o[2]);
-    float2 o3 = make_float2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
-    v[0] = cadd(e[0], o[0]);
-    v[4] = csub(e[0], o[0]);
-    v[1] = cadd(e[1], o1);
-    v[5] = csub(e[1], o1);
-    v[2] = cadd(e[2], o2);
-    v[6] = csub(e[2], o2);
-    v[3] = cadd(e[3], o3);
-    v[7] = csub(e[3], o3);
-}
-// 16 points as two interleaved 8-point transforms and the e^{-2 pi i k / 16} butterflies
-__device__ inline void dft16(float2* v) {
-    const float c1 = 0.92387953251128675613f;   // cos(pi / 8)
-    const float s1 = 0.38268343236508977173f;   // sin(pi / 8)
-    const float r = 0.70710678118654752440f;
-    float2 e[8], o[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        e[i] = v[2 * i];
-        o[i] = v[2 * i + 1];
-    }
-    dft8(e);
-    dft8(o);
-    float2 t[8];
-    t[0] = o[0];
-    t[1] = make_float2(c1 * o[1].x + s1 * o[1].y, c1 * o[1].y - s1 * o[1].x);
-    t[2] = make_float2(r * (o[2].x + o[2].y), r * (o[2].y - o[2].x));
-    t[3] = make_float2(s1 * o[3].x + c1 * o[3].y, s1 * o[3].y - c1 * o[3].x);
-    t[4] = cmul_mi(o[4]);
-    t[5] = make_float2(c1 * o[5].y - s1 * o[5].x, -(s1 * o[5].y + c1 * o[5].x));
-    t[6] = make_float2(r * (o[6].y - o[6].x), -r * (o[6].x + o[6].y));
-    t[7] = make_float2(s1 * o[7].y - c1 * o[7].x, -(c1 * o[7].y + s1 * o[7].x));
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        v[k] = cadd(e[k], t[k]);
-        v[k + 8] = csub(e[k], t[k]);
-    }
-}
-template <int R>
-__device__ inline void dft(float2* v) {
-    if constexpr (R == 2) dft2(v);
-    else if constexpr (R == 4) dft4(v);
-    else if constexpr (R == 8) dft8(v);
-    else dft16(v);
-}
-
-// radix of the Stockham pass at sub-length Ns: 8 where the remaining length
-// is a power of 8, otherwise a 16 first so that the passes stay at
-// ceil(log2(left) / 3) or fewer (Q = 8192 = 16 * 8^3, 2048 = 16 * 16 * 8);
-// 4096 = 16^3 (three passes, measured 2% faster than 8^4)
-constexpr int pick_radix(int left) {
-    return left <= 2 ? left : left == 4 ? 4 : (left == 8 || left == 32) ? 8
-         : left == 4096 ? 16 : (ilog2c(left) % 3 == 0 ? 8 : 16);
-}
-
-// e^{2 pi i rev} from the hardware sine/cosine (argument in revolutions;
-// absolute error 1.3e-7 for |rev| <= 1.5, measured on gfx950)
-__device__ inline float2 cis_rev(float rev) {
-    return make_float2(__builtin_amdgcn_cosf(rev), __builtin_amdgcn_sinf(rev));
-}
-
-// Workgroup geometry of class Q: NSL slots of Q complex points in LDS,
-// 16 points per thread.
-template <int Q>
-struct Geo {
-    static constexpr int NSL = Q >= 2048 ? 2 : (4096 / Q > 512 ? 512 : 4096 / Q);
-    static constexpr int NPTS = NSL * Q;
-    static constexpr int EPT = 16;
-    static constexpr int NT = NPTS / EPT;
-    static constexpr int NCELL = NSL / 2;
-};
-
-// column kernel modes: one channel; channel sum of products (MMP3 and
-// multichannel workspaces); MMP1 constraint (three accumulators per bin)
-enum { kModeSingle = 0, kModeSum = 1, kModeMmp1 = 2 };
-// waves per SIMD the register allocator must allow in the multichannel modes
-template <int Q, int MODE>
-constexpr int column_waves() { return MODE == kModeSum ? 2 : 1; }
-
-// Tuning variants of the single-channel column kernels: waves per SIMD the
-// register allocator must allow (4, 3, 2), times two persistent grid sizes.
-// Q = 8192 fills a CU's LDS with one workgroup: always 2 waves.
-constexpr int kWaveVariants = 3;
-constexpr int kGridVariants = 2;
-constexpr int kTuneChoices = kWaveVariants * kGridVariants;
-template <int Q, int V>
-constexpr int variant_waves() { return Q >= 8192 ? 4 : (V == 0 ? 4 : V == 1 ? 3 : 2); }
-
-// E-table entries staged in LDS per work item (scales with more fall back to HBM)
-constexpr int kEtabLds = 128;
-
-// LDS index with one float2 of padding per 8: the stride-8 writes of the
-// first radix-8 passes then spread over all banks.
-__device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> 3); }
-template <int NPTS>
-struct Lds {
-    static constexpr int SIZE = NPTS + NPTS / 8;
-};
-
-// One Stockham autosort pass of radix R at sub-length Ns over all slots.
-template <int Q, int NT, int NPTS, int Ns, int R>
-__device__ inline void stockham_pass(float2* buf) {
-    constexpr int BPS = Q / R;
-    constexpr int NB = NPTS / R;
-    constexpr int PER = NB / NT;
-    static_assert(PER * NT == NB, "pass does not tile the workgroup");
-    const int tid = threadIdx.x;
-    float2 v[PER][R];
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int bi = tid + b * NT;
-        const int slot = bi / BPS, j = bi % BPS;
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = buf[lpad(slot * Q + j + r * BPS)];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-        const int bi = tid + b * NT;
-        const int slot = bi / BPS, j = bi % BPS;
-        int k = j & (Ns - 1);
-        if constexpr (Ns > 1) {
-            // twiddles are recomputed per pass rather than hoisted out of the
-            // persistent loop: holding them for every pass costs more
-            // registers than the hardware sine/cosine costs cycles
-            asm volatile("" : "+v"(k));
-            // w^r, w = e^{-2 pi i k / (Ns R)}, by products of the hardware w (depth <= 3)
-            float2 w[R];
-            w[1] = cis_rev(-static_cast<float>(k) * (1.0f / static_cast<float>(Ns * R)));
-            if constexpr (R >= 4) {
-                w[2] = cmul(w[1], w[1]);
-                w[3] = cmul(w[2], w[1]);
-            }
-            if constexpr (R >= 8) {
-                w[4] = cmul(w[2], w[2]);
-                w[5] = cmul(w[4], w[1]);
-                w[6] = cmul(w[3], w[3]);
-                w[7] = cmul(w[4], w[3]);
-            }
-            if constexpr (R == 16) {
-#pragma unroll
-                for (int r = 8; r < 16; ++r) w[r] = cmul(w[r / 2], w[r - r / 2]);   // depth <= 4
-            }
-#pragma unroll
-            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r]);
-        }
-        dft<R>(v[b]);
-        const int base = slot * Q + (j - k) * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) buf[lpad(base + r * Ns)] = v[b][r];
-    }
-    __syncthreads();
-}
-
-template <int Q, int NT, int NPTS, int Ns>
-__device__ inline void fft_passes(float2* buf) {
-    if constexpr (Ns < Q) {
-        constexpr int left = Q / Ns;
-        constexpr int R = pick_radix(left);
-        stockham_pass<Q, NT, NPTS, Ns, R>(buf);
-        fft_passes<Q, NT, NPTS, Ns * R>(buf);
-    }
-}
-
-__device__ inline unsigned long long pack_best(float p, unsigned bin) {
-    if (!(p > 0.0f)) p = 0.0f;   // NaN and negatives never win
-    return (static_cast<unsigned long long>(__float_as_uint(p)) << 32) | (0xFFFFFFFFu - bin);
-}
-
-// sub-transform indices of a cell: g = 0 -> {0, P/2}, g > 0 -> {g, P - g}; -1 = none
-__device__ inline void cell_k2(int g, int P, int* ka, int* kb) {
-    if (g == 0) {
-        *ka = 0;
-        *kb = P >= 2 ? P / 2 : -1;
-    } else {
-        *ka = g;
-        *kb = P - g;
-    }
-}
-
-// ------------------------------------------------------------ column kernel
-
-struct CellInfo {
-    int valid;      // re-evaluated by this pass
-    int exists;     // cell < T*G
-    int iL;
-    int W;
-    int dint;
-    float dfrac;
-    int cell;
-    int ka, kb;     // sub-transforms of the cell's two slots, -1 = none
-};
-
-// e^{2 pi i cyc} from the hardware sine/cosine, whose argument is in
-// revolutions (absolute error 1.3e-7 for |cyc| <= 1.5, measured on gfx950)
-__device__ inline void twiddle_rev(float cyc, float* sn, float* cs) {
-    *sn = __builtin_amdgcn_sinf(cyc);
-    *cs = __builtin_amdgcn_cosf(cyc);
-}
-
-__device__ inline unsigned long long shfl_xor_u64(unsigned long long v, int m) {
-    const unsigned lo = __shfl_xor(static_cast<unsigned>(v), m);
-    const unsigned hi = __shfl_xor(static_cast<unsigned>(v >> 32), m);
-    return (static_cast<unsigned long long>(hi) << 32) | lo;
-}
 
 // Affected tiles of every job of one class (one workgroup per class), one
 // work item per job: columns with |t - tA| <= 3 (s + sA), widened by one
@@ -367,398 +125,6 @@ __global__ __launch_bounds__(kPlanThreads) void k_scan_items(const ClassDev* __r
     const ClassDev cd = classes[blockIdx.x];
     const int n = min(*cd.count, cd.max_items);
     scan_items(cd, n, part);
-}
-
-// MODE kModeSingle: one channel, no constraint (SMP units, MMP2 sums);
-// kModeSum: sum over channels of the products (MMP3); kModeMmp1: MMP1
-// constraint on low-frequency bins, channel sum on high ones.
-// Persistent: workgroups stride through the class's affected tiles.
-template <int Q, int MODE, int W>
-__global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W))) void k_columns(
-    const ClassDev* __restrict__ classes, int cls, const ScaleDev* __restrict__ scales,
-    const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
-    const float* __restrict__ residual, unsigned long long* __restrict__ cellbest,
-    unsigned long long* __restrict__ tilebest,
-    unsigned long long* __restrict__ counter) {
-    using G_ = Geo<Q>;
-    constexpr int NT = G_::NT, NPTS = G_::NPTS, NCELL = G_::NCELL, EPT = G_::EPT;
-    constexpr bool MULTI = MODE != kModeSingle;
-    constexpr int NACC = MULTI ? EPT : 1;
-    constexpr int NACC3 = MODE == kModeMmp1 ? EPT : 1;
-    constexpr int SEG = 2 * Q < 64 ? 2 * Q : 64;          // lanes sharing a cell
-    constexpr int GRP = 2 * Q >= NT ? 2 * Q / NT : 1;      // consecutive elements sharing a cell
-    static_assert(EPT % GRP == 0, "element groups");
-    __shared__ float2 buf[Lds<NPTS>::SIZE];
-    __shared__ CellInfo info[NCELL];
-    __shared__ unsigned long long best[NCELL];
-    __shared__ unsigned long long tbest;
-    __shared__ unsigned cnt_lds;
-    __shared__ float2 etl[kEtabLds + 1];    // ((1 - E)/K', 2E/K') of the item's scale; [kEtabLds] = (1/K', 0)
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const ClassDev cd = classes[cls];
-    const int nitems = min(*cd.count, cd.max_items);
-    const int total = cd.work[nitems];
-    unsigned long long evaluated = 0, evaluated_ch = 0;
-
-    // XCD-aware walk: workgroups are dealt round-robin over the 8 XCDs, so
-    // workgroup b runs on XCD b % 8; each XCD takes one contiguous eighth of
-    // the work list, keeping neighbouring cells (overlapping windows) in one L2
-    constexpr int kXcds = 8;
-    const int xcd = blockIdx.x % kXcds;
-    const int per_xcd = gridDim.x / kXcds;
-    const int chunk = (total + kXcds - 1) / kXcds;
-    const int w_end = min(total, (xcd + 1) * chunk);
-    for (int w = xcd * chunk + static_cast<int>(blockIdx.x / kXcds); w < w_end; w += per_xcd) {
-        // item: last i with work[i] <= w
-        int lo = 0, hi = nitems - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (cd.work[mid] <= w) lo = mid; else hi = mid - 1;
-        }
-        const WorkItem item = cd.items[lo];
-        const Job job = cd.jobs[item.job];
-        const int tile = item.tile + (w - cd.work[lo]);
-        const UnitState st = state[job.unit];
-        const ScaleDev sd = scales[job.scale];
-        const UnitDev ud = units[job.unit];
-        const int cell0 = tile * NCELL;
-        const int ncells = sd.T * sd.G;
-        const double s = sd.s, fs = sd.fs;
-        const int P = sd.P, G = sd.G, M = sd.M, F = sd.F, Nfft = sd.Nfft;
-
-        for (int c = tid; c < NCELL; c += NT) {
-            CellInfo ci{};
-            ci.cell = cell0 + c;
-            ci.exists = ci.cell < ncells;
-            ci.valid = ci.exists;
-            ci.kb = -1;
-            if (ci.exists) {
-                const int t = ci.cell / G;
-                const int g = ci.cell - t * G;
-                const double t0 = t * sd.tMax / (sd.T - 1);
-                ci.valid = t >= item.tlo && t <= item.thi;
-                if (item.atom) ci.valid = ci.valid && fabs(t0 - st.tA) <= (s + st.sA) * kGaussHalfWidth;
-                const double hw = kGaussHalfWidth * s;
-                const long l = lrint((t0 - hw) * fs);
-                const long r = lrint((t0 + hw) * fs);
-                ci.iL = static_cast<int>(l > 0 ? l : 0);
-                const int iR = static_cast<int>(r < sd.N - 1 ? r : sd.N - 1);
-                ci.W = min(iR - ci.iL + 1, 2 * Q);
-                const double d = (t0 - ci.iL / fs) * fs;
-                const double dfl = floor(d);
-                ci.dint = static_cast<int>(dfl);
-                ci.dfrac = static_cast<float>(d - dfl);
-                cell_k2(g, P, &ci.ka, &ci.kb);
-            }
-            info[c] = ci;
-            best[c] = 0ull;
-        }
-        if (tid == 0) {
-            cnt_lds = 0;
-            tbest = 0ull;
-        }
-        for (int i = tid; i <= kEtabLds; i += NT) etl[i] = i < sd.JT ? sd.etab[i] : make_float2(sd.invKp, 0.0f);
-        int any = 0;
-        for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
-        if (!__syncthreads_or(any)) continue;
-
-        const float invN = ldexpf(1.0f, -sd.lgNfft);   // Nfft is a power of two
-        const float invM = 2.0f * invN;
-        constexpr bool mmp1 = MODE == kModeMmp1;
-        const int C = MULTI ? ud.C : 1;
-
-        float acc0[NACC], acc1[NACC3], acc2[NACC3];
-        float accNyq = 0.0f;
-#pragma unroll
-        for (int i = 0; i < NACC; ++i) acc0[i] = 0.0f;
-#pragma unroll
-        for (int i = 0; i < NACC3; ++i) acc1[i] = acc2[i] = 0.0f;
-        unsigned mycnt = 0;
-
-        for (int ch = 0; ch < C; ++ch) {
-            const float* x = residual + ud.res_off + static_cast<long long>(ch) * ud.N;
-            // load: packed pairs z[m] = w[2m] + i w[2m+1], pre-twiddled per slot
-            constexpr int GL = Q >= NT ? Q / NT : 1;   // consecutive pairs sharing a cell
-#pragma unroll
-            for (int gi = 0; gi < EPT / 2 / GL; ++gi) {
-                const int c = (tid + gi * GL * NT) / Q;
-                const CellInfo ci = info[c];
-#pragma unroll 2
-                for (int q = 0; q < GL; ++q) {
-                    const int m = (tid + (gi * GL + q) * NT) % Q;
-                    const int n0 = 2 * m, n1 = 2 * m + 1;
-                    // unconditional loads at a clamped index: no branch around
-                    // them, so the unrolled iterations' loads overlap
-                    const bool v0 = ci.valid && n0 < ci.W, v1 = ci.valid && n1 < ci.W;
-                    const float x0 = x[v0 ? ci.iL + n0 : 0];
-                    const float x1 = x[v1 ? ci.iL + n1 : 0];
-                    float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
-                    if (ci.valid) {
-                        float w0 = 0.f, w1 = 0.f;
-                        if (v0) {
-                            const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
-                            w0 = x0 * __expf(-sd.gA * u * u);
-                        }
-                        if (v1) {
-                            const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
-                            w1 = x1 * __expf(-sd.gA * u * u);
-                        }
-                        const float2 z = make_float2(w0, w1);
-                        // pre-twiddle w_M^{k2 m}, k2 m < M: exact revolution counts
-                        const unsigned um = static_cast<unsigned>(m);   // k2 m < M < 2^24
-                        za = ci.ka == 0 ? z
-                                        : cmul(z, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.ka), um)) * invM));
-                        if (ci.kb >= 0)
-                            zb = cmul(z, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.kb), um)) * invM));
-                    }
-                    buf[lpad((2 * c) * Q + m)] = za;
-                    buf[lpad((2 * c + 1) * Q + m)] = zb;
-                }
-            }
-            __syncthreads();
-            fft_passes<Q, NT, NPTS, 1>(buf);
-
-            // post: split the packed spectrum, phase-correct, product per bin.
-            // element(rem, c, ci, i, run): bin `rem` (of 2Q) of cell c; i indexes
-            // the multichannel accumulators.
-            // product of bin k from its spectrum value X = Xr + i Xi
-            auto from_x = [&](const int k, const float Xr, const float Xi, const CellInfo& ci, const int i,
-                              unsigned long long& run) {
-                const bool high = 4 * k > Nfft;
-                const int j = high ? M - k : k;
-                const float num = Xr * Xr + Xi * Xi;
-                // Beyond JT the window term E is zero and P = K' |X|^2: the
-                // phase of u = X e^{2 pi i cyc} enters only the first bins
-                // (and MMP1's constraint on the low half).
-                const bool edge = j < sd.JT;
-                float a = 0.0f, b = 0.0f;
-                if (edge || (mmp1 && !high)) {
-                    // j, dint < 2^24: the 24-bit product is exact modulo 2^32, and Nfft divides 2^32
-                    const unsigned ci_int = __umul24(static_cast<unsigned>(j), static_cast<unsigned>(ci.dint)) &
-                                            static_cast<unsigned>(Nfft - 1);
-                    float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
-                    if (high) cyc = -cyc - 0.5f * static_cast<float>(ci.iL & 1);
-                    float sn, cs;
-                    twiddle_rev(cyc, &sn, &cs);
-                    a = Xr * cs - Xi * sn;
-                    b = Xr * sn + Xi * cs;
-                }
-                float pe = num * sd.Kp;   // the product where E = 0
-                if (edge) {
-                    float2 tt = etl[min(j, kEtabLds)];
-                    if (j >= kEtabLds) tt = sd.etab[j];
-                    const float den = fmaf(tt.x, num, tt.y * a * a);
-                    pe = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
-                }
-                if constexpr (!MULTI) {
-                    ++mycnt;
-                    run = max(run, pack_best(pe, static_cast<unsigned>(k)));
-                } else if (mmp1 && !high) {
-                    if constexpr (mmp1) {
-                        acc0[i] += a * a;
-                        acc1[i] += b * b;
-                        acc2[i] += a * b;
-                    }
-                } else {
-                    acc0[i] += pe;
-                }
-            };
-            auto element = [&](const int rem, const int c, const CellInfo& ci, const int i,
-                               unsigned long long& run) {
-                const int half = rem / Q, k1 = rem % Q;
-                const int k2 = half ? ci.kb : ci.ka;
-                const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
-                if (!(ci.valid && k2 >= 0 && k < F)) return;
-                const int slot = 2 * c + half;
-                const float2 Zk = buf[lpad(slot * Q + k1)];
-                float2 Zp;
-                if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
-                else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
-                const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
-                const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
-                const float2 wv = cis_rev(-static_cast<float>(k) * invN);
-                const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
-                const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
-                from_x(k, Xr, Xi, ci, i, run);
-            };
-            // Two bins from one pair of LDS values (single-channel units,
-            // Q >= NT): bin k and its mirror M - k share A = Z[k] + conj Z[M-k],
-            // B = Z[k] - conj Z[M-k] and w^k: X[k] = (A - i w^k B) / 2,
-            // X[M-k] = conj(A + i w^k B) / 2.  Pair u < Q of cell c is (slot a, u)
-            // with (slot b, Q-1-u) for g > 0; for g = 0, (a, u) with (a, Q-u) for
-            // u < Q/2 and (b, u') with (b, Q-1-u'), u' = u - Q/2; u = 0 also takes
-            // the self-mirrored (a, Q/2).  The Nyquist bin keeps its own path.
-            auto pair = [&](const int u, const int c, const CellInfo& ci, unsigned long long& run) {
-                if (!ci.valid) return;
-                int sa, ka1, sb, kb1, k2;
-                if (ci.ka != 0) {
-                    sa = 2 * c; ka1 = u; sb = 2 * c + 1; kb1 = Q - 1 - u; k2 = ci.ka;
-                } else if (u < Q / 2) {
-                    sa = sb = 2 * c; ka1 = u; kb1 = (Q - u) & (Q - 1); k2 = 0;
-                } else {
-                    if (ci.kb < 0) return;
-                    sa = sb = 2 * c + 1; ka1 = u - Q / 2; kb1 = Q - 1 - ka1; k2 = ci.kb;
-                }
-                const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(ka1))) + k2;
-                const int km = M - k;
-                const float2 Zk = buf[lpad(sa * Q + ka1)];
-                const float2 Zp = buf[lpad(sb * Q + kb1)];
-                const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
-                const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
-                const float2 wv = cis_rev(-static_cast<float>(k) * invN);
-                const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
-                if (k < F) from_x(k, 0.5f * (Ar + pi), 0.5f * (Ai - pr), ci, 0, run);
-                if (km < F && km != M) from_x(km, 0.5f * (Ar - pi), -0.5f * (Ai + pr), ci, 0, run);
-                if (ci.ka == 0 && u == 0 && M / 2 < F) {
-                    // (a, Q/2): bin M/2, its own mirror (Z[k] = Z[M-k])
-                    const float2 Zh = buf[lpad(2 * c * Q + Q / 2)];
-                    const float2 wh = cis_rev(-static_cast<float>(M / 2) * invN);
-                    const float hBi = Zh.y + Zh.y;                 // B = (0, 2 Im Zh), A = (2 Re Zh, 0)
-                    const float hpr = -wh.y * hBi, hpi = wh.x * hBi;
-                    from_x(M / 2, 0.5f * (Zh.x + Zh.x + hpi), -0.5f * hpr, ci, 0, run);
-                }
-            };
-            constexpr bool CONTIG = !MULTI && Q >= 8 && Q <= 256;
-            if constexpr (CONTIG) {
-                // Small sub-transforms: a thread's EPT elements are consecutive
-                // bins of one cell, visited in a rotated order that spreads the
-                // LDS banks, so each cell needs one cross-lane reduction per tile.
-                constexpr int TPC = 2 * Q / EPT;   // threads per cell
-                const int c = tid / TPC;
-                const CellInfo ci = info[c];
-                const int r0 = (tid % TPC) * EPT;
-                unsigned long long run = 0ull;
-#pragma unroll 2
-                for (int s2 = 0; s2 < EPT; ++s2) element(r0 + ((s2 + tid) & (EPT - 1)), c, ci, 0, run);
-#pragma unroll
-                for (int off = TPC / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
-                if ((tid & (TPC - 1)) == 0 && run) atomicMax(&best[c], run);
-            } else {
-                // elements of one group share a cell: its info is read once
-#pragma unroll MULTI ? EPT / GRP : 1
-                for (int gi = 0; gi < EPT / GRP; ++gi) {
-                    const int c = (tid + gi * GRP * NT) / (2 * Q);
-                    const CellInfo ci = info[c];
-                    unsigned long long run = 0ull;
-                    if constexpr (MULTI) {
-#pragma unroll
-                        for (int q = 0; q < GRP; ++q) {
-                            const int i = gi * GRP + q;
-                            element((tid + i * NT) % (2 * Q), c, ci, i, run);
-                            __builtin_amdgcn_sched_barrier(0);   // keep per-element live ranges short
-                        }
-                    } else {
-                        if constexpr (GRP >= 2) {
-#pragma unroll 2
-                            for (int q = 0; q < GRP / 2; ++q) pair(tid + q * NT, c, ci, run);
-                        } else {
-                            element((tid + gi * NT) % (2 * Q), c, ci, gi, run);
-                        }
-#pragma unroll
-                        for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
-                        if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
-                    }
-                }
-            }
-            // Nyquist bin k = M of g = 0 cells: X[M] = Re Z0 - Im Z0, u real, E = 1
-            if (tid < NCELL && F > M) {
-                const CellInfo& ci = info[tid];
-                if (ci.valid && ci.ka == 0) {
-                    const float2 Z0 = buf[lpad((2 * tid) * Q)];
-                    const float a = Z0.x - Z0.y;
-                    accNyq += a * a;
-                }
-            }
-            __syncthreads();
-        }
-
-        // final products of multi-channel units and per-cell maximum
-        if constexpr (MULTI) {
-#pragma unroll
-            for (int gi = 0; gi < EPT / GRP; ++gi) {
-                const int c = (tid + gi * GRP * NT) / (2 * Q);
-                const CellInfo ci = info[c];
-                unsigned long long run = 0ull;
-#pragma unroll
-                for (int q = 0; q < GRP; ++q) {
-                    const int i = gi * GRP + q;
-                    const int e = tid + i * NT;
-                    const int rem = e % (2 * Q);
-                    const int half = rem / Q, k1 = rem % Q;
-                    const int k2 = half ? ci.kb : ci.ka;
-                    const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
-                    if (!(ci.valid && k2 >= 0 && k < F)) continue;
-                    ++mycnt;
-                    const bool high = 4 * k > Nfft;
-                    float pv;
-                    if (mmp1 && !high) {
-                        float Srr = acc0[i], Sii = 0.0f, Sri = 0.0f;
-                        if constexpr (mmp1) {
-                            Sii = acc1[i];
-                            Sri = acc2[i];
-                        }
-                        const float Dr = Srr - Sii, Di = 2.0f * Sri;
-                        const float Dm = hypotf(Dr, Di);
-                        const float lam = 0.5f * (Srr + Sii + Dm);
-                        float cos2 = 1.0f;
-                        if (Dm > 0.0f) cos2 = Dr >= 0.0f ? (Dm + Dr) / (2.0f * Dm) : (Di * Di) / (2.0f * Dm * (Dm - Dr));
-                        float2 tt = etl[min(k, kEtabLds)];
-                        if (sd.JT > kEtabLds && k >= kEtabLds && k < sd.JT) tt = sd.etab[k];
-                        const float T1 = tt.x, T2 = tt.y;
-                        const float den = fmaf(T2, cos2, T1);
-                        pv = den > 0.0f ? lam / den : 0.0f;
-                    } else {
-                        pv = acc0[i];
-                    }
-                    run = max(run, pack_best(pv, static_cast<unsigned>(k)));
-                }
-#pragma unroll
-                for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
-                if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
-            }
-        }
-        if (tid < NCELL && F > M) {
-            const CellInfo& ci = info[tid];
-            if (ci.valid && ci.ka == 0) {
-                ++mycnt;
-                // E = 1 at Nyquist: |u|^4 / (2 E a^2 / K') = K' a^2 / 2 summed over channels
-                atomicMax(&best[tid], pack_best(0.5f * sd.Kp * accNyq, static_cast<unsigned>(M)));
-            }
-        }
-        if (mycnt) atomicAdd(&cnt_lds, mycnt);
-        __syncthreads();
-        // cell bests out, tile best over re-evaluated and untouched cells
-        unsigned long long tb = 0ull;
-        for (int c = tid; c < NCELL; c += NT) {
-            const CellInfo& ci = info[c];
-            if (!ci.exists) continue;
-            unsigned long long cb;
-            if (ci.valid) {
-                cb = best[c];
-                cellbest[job.cell_off + ci.cell] = cb;
-            } else {
-                cb = cellbest[job.cell_off + ci.cell];
-            }
-            const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(cb);
-            const unsigned key = (bin << 10) | static_cast<unsigned>(c);
-            tb = max(tb, (cb & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFu - key));
-        }
-        if (tb) atomicMax(&tbest, tb);
-        __syncthreads();
-        if (tid == 0) {
-            tilebest[job.tile_off + tile] = tbest;
-            evaluated += cnt_lds;
-            evaluated_ch += static_cast<unsigned long long>(cnt_lds) * (MULTI ? ud.C : 1);
-        }
-        __syncthreads();
-    }
-    if (tid == 0 && counter && evaluated) {
-        constexpr int cls_ix = ilog2c(Q) - 1;
-        atomicAdd(&counter[2 * cls_ix], evaluated);
-        atomicAdd(&counter[2 * cls_ix + 1], evaluated_ch);
-    }
 }
 
 // ------------------------------------------------------------ unit kernels
@@ -1707,21 +1073,6 @@ struct ItemInfo {
     long long scratch_off;    // MMP2: refit scratch [C][N]
 };
 
-constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
-
-typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
-                               const UnitDev* units, const UnitState* state, const float* residual,
-                               unsigned long long* cellbest, unsigned long long* tilebest,
-                               unsigned long long* counter);
-
-template <int Q, int MODE, int W>
-void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
-                    const UnitDev* units, const UnitState* state, const float* residual,
-                    unsigned long long* cellbest, unsigned long long* tilebest, unsigned long long* counter) {
-    hipLaunchKernelGGL((k_columns<Q, MODE, W>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
-                       state, residual, cellbest, tilebest, counter);
-}
-
 // persistent grid: enough workgroups to fill 256 CUs a few times over (twice
 // that with grid variant 1), a multiple of the 8 XCDs, which the kernel's
 // work walk relies on
@@ -1732,32 +1083,10 @@ int column_grid(int Q, long long tiles, int gv) {
     return static_cast<int>((g + 7) / 8 * 8);
 }
 
-#define EMPI_SINGLE_ROW(V)                                                                                   \
-    {launch_columns<2, 0, variant_waves<2, V>()>, launch_columns<4, 0, variant_waves<4, V>()>,               \
-     launch_columns<8, 0, variant_waves<8, V>()>, launch_columns<16, 0, variant_waves<16, V>()>,             \
-     launch_columns<32, 0, variant_waves<32, V>()>, launch_columns<64, 0, variant_waves<64, V>()>,           \
-     launch_columns<128, 0, variant_waves<128, V>()>, launch_columns<256, 0, variant_waves<256, V>()>,       \
-     launch_columns<512, 0, variant_waves<512, V>()>, launch_columns<1024, 0, variant_waves<1024, V>()>,     \
-     launch_columns<2048, 0, variant_waves<2048, V>()>, launch_columns<4096, 0, variant_waves<4096, V>()>,   \
-     launch_columns<8192, 0, variant_waves<8192, V>()>}
-#define EMPI_MULTI_ROW(M)                                                                                    \
-    {launch_columns<2, M, column_waves<2, M>()>, launch_columns<4, M, column_waves<4, M>()>,                 \
-     launch_columns<8, M, column_waves<8, M>()>, launch_columns<16, M, column_waves<16, M>()>,               \
-     launch_columns<32, M, column_waves<32, M>()>, launch_columns<64, M, column_waves<64, M>()>,             \
-     launch_columns<128, M, column_waves<128, M>()>, launch_columns<256, M, column_waves<256, M>()>,         \
-     launch_columns<512, M, column_waves<512, M>()>, launch_columns<1024, M, column_waves<1024, M>()>,       \
-     launch_columns<2048, M, column_waves<2048, M>()>, launch_columns<4096, M, column_waves<4096, M>()>,     \
-     launch_columns<8192, M, column_waves<8192, M>()>}
-
-// single-channel kernels [wave variant][class]; multi-channel [mode - 1][class]
-const ColumnLauncher kSingleLaunchers[kWaveVariants][kNumClasses] = {EMPI_SINGLE_ROW(0), EMPI_SINGLE_ROW(1),
-                                                                      EMPI_SINGLE_ROW(2)};
-const ColumnLauncher kMultiLaunchers[2][kNumClasses] = {EMPI_MULTI_ROW(1), EMPI_MULTI_ROW(2)};
-#undef EMPI_SINGLE_ROW
-#undef EMPI_MULTI_ROW
-
 ColumnLauncher column_launcher(int mode, int cls, int choice) {
-    return mode == kModeSingle ? kSingleLaunchers[choice % kWaveVariants][cls] : kMultiLaunchers[mode - 1][cls];
+    const int lg = cls + kMinLgQ, v = choice % kWaveVariants;
+    return lg <= 6 ? column_launcher_small(lg, mode, v)
+         : lg <= 10 ? column_launcher_mid(lg, mode, v) : column_launcher_large(lg, mode, v);
 }
 
 // default tuning: wave variant per class measured on the 64 x 20 x 5120 SMP
