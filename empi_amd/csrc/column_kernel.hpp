@@ -75,8 +75,16 @@ __device__ inline void stockham_pass(float2* buf) {
         }
         dft<R>(v[b]);
         const int base = slot * Q + (j - k) * R + k;
+        if constexpr (Ns % 8 == 0) {
+            // 8 | r Ns: lpad(base + r Ns) = lpad(base) + r (Ns + Ns / 8), one
+            // address and constant offsets
+            const int pb = lpad(base);
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[lpad(base + r * Ns)] = v[b][r];
+            for (int r = 0; r < R; ++r) buf[pb + r * (Ns + Ns / 8)] = v[b][r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[lpad(base + r * Ns)] = v[b][r];
+        }
     }
     __syncthreads();
 }
