@@ -148,6 +148,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     __shared__ unsigned long long tbest;
     __shared__ unsigned cnt_lds;
     __shared__ float2 etl[kEtabLds + 1];    // ((1 - E)/K', 2E/K') of the item's scale; [kEtabLds] = (1/K', 0)
+    // windows longer than 2Q samples (Q = kMaxQ only): e^{-2 pi i k2 / P} of each cell's two slots
+    __shared__ float2 ftw[Q == kMaxQ ? 2 * NCELL : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -199,12 +201,17 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const long r = lrint((t0 + hw) * fs);
                 ci.iL = static_cast<int>(l > 0 ? l : 0);
                 const int iR = static_cast<int>(r < sd.N - 1 ? r : sd.N - 1);
-                ci.W = min(iR - ci.iL + 1, 2 * Q);
+                ci.W = min(iR - ci.iL + 1, 2 * Q * sd.fold);
                 const double d = (t0 - ci.iL / fs) * fs;
                 const double dfl = floor(d);
                 ci.dint = static_cast<int>(dfl);
                 ci.dfrac = static_cast<float>(d - dfl);
                 cell_k2(g, P, &ci.ka, &ci.kb);
+            }
+            if constexpr (Q == kMaxQ) {
+                ftw[2 * c] = cis_rev(-static_cast<float>(ci.ka) / static_cast<float>(P));
+                ftw[2 * c + 1] = ci.kb >= 0 ? cis_rev(-static_cast<float>(ci.kb) / static_cast<float>(P))
+                                            : make_float2(0.0f, 0.0f);
             }
             info[c] = ci;
             best[c] = 0ull;
@@ -233,7 +240,26 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
 
         for (int ch = 0; ch < C; ++ch) {
             const float* x = residual + ud.res_off + static_cast<long long>(ch) * ud.N;
-            // load: packed pairs z[m] = w[2m] + i w[2m+1], pre-twiddled per slot
+            // load: packed pairs z[m] = w[2m] + i w[2m+1], pre-twiddled per slot.
+            // Windowed pair mm of a cell; samples beyond the window are zero.
+            // Unconditional loads at a clamped index: no branch around them,
+            // so the unrolled iterations' loads overlap.
+            auto window_pair = [&](const CellInfo& ci, const int mm) {
+                const int n0 = 2 * mm, n1 = 2 * mm + 1;
+                const bool v0 = ci.valid && n0 < ci.W, v1 = ci.valid && n1 < ci.W;
+                const float x0 = x[v0 ? ci.iL + n0 : 0];
+                const float x1 = x[v1 ? ci.iL + n1 : 0];
+                float w0 = 0.f, w1 = 0.f;
+                if (v0) {
+                    const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
+                    w0 = x0 * __expf(-sd.gA * u * u);
+                }
+                if (v1) {
+                    const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
+                    w1 = x1 * __expf(-sd.gA * u * u);
+                }
+                return make_float2(w0, w1);
+            };
             constexpr int GL = Q >= NT ? Q / NT : 1;   // consecutive pairs sharing a cell
 #pragma unroll
             for (int gi = 0; gi < EPT / 2 / GL; ++gi) {
@@ -242,30 +268,34 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
 #pragma unroll 2
                 for (int q = 0; q < GL; ++q) {
                     const int m = (tid + (gi * GL + q) * NT) % Q;
-                    const int n0 = 2 * m, n1 = 2 * m + 1;
-                    // unconditional loads at a clamped index: no branch around
-                    // them, so the unrolled iterations' loads overlap
-                    const bool v0 = ci.valid && n0 < ci.W, v1 = ci.valid && n1 < ci.W;
-                    const float x0 = x[v0 ? ci.iL + n0 : 0];
-                    const float x1 = x[v1 ? ci.iL + n1 : 0];
+                    float2 z = window_pair(ci, m);
                     float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
                     if (ci.valid) {
-                        float w0 = 0.f, w1 = 0.f;
-                        if (v0) {
-                            const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
-                            w0 = x0 * __expf(-sd.gA * u * u);
+                        float2 zin_b = z;
+                        if constexpr (Q == kMaxQ) {
+                            // input m + f Q enters sub-transform k2 with e^{-2 pi i k2 f / P}
+                            if (sd.fold > 1) {
+                                const float2 ta = ftw[2 * c], tb = ftw[2 * c + 1];
+                                float2 wa = ta, wb = tb, sa = z, sb = z;
+                                for (int f = 1; f < sd.fold; ++f) {
+                                    if (f > 1) {
+                                        wa = cmul(wa, ta);
+                                        wb = cmul(wb, tb);
+                                    }
+                                    const float2 zf = window_pair(ci, m + f * Q);
+                                    sa = cadd(sa, cmul(zf, wa));
+                                    sb = cadd(sb, cmul(zf, wb));
+                                }
+                                z = sa;
+                                zin_b = sb;
+                            }
                         }
-                        if (v1) {
-                            const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
-                            w1 = x1 * __expf(-sd.gA * u * u);
-                        }
-                        const float2 z = make_float2(w0, w1);
                         // pre-twiddle w_M^{k2 m}, k2 m < M: exact revolution counts
                         const unsigned um = static_cast<unsigned>(m);   // k2 m < M < 2^24
                         za = ci.ka == 0 ? z
                                         : cmul(z, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.ka), um)) * invM));
                         if (ci.kb >= 0)
-                            zb = cmul(z, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.kb), um)) * invM));
+                            zb = cmul(zin_b, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.kb), um)) * invM));
                     }
                     buf[lpad((2 * c) * Q + m)] = za;
                     buf[lpad((2 * c + 1) * Q + m)] = zb;

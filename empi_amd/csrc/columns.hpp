@@ -1,4 +1,4 @@
-// FFT-size classes of the column kernels (Q = 2 .. 8192 complex points per
+// FFT-size classes of the column kernels (Q = 2 .. 4096 complex points per
 // slot), their workgroup geometry, modes and tuning variants, and the launch
 // table the host plan dispatches through.  The kernels are compiled in three
 // translation units by class range; each exports one launcher lookup.
@@ -6,13 +6,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dictionary.hpp"
 #include "kernels.hpp"
 
 namespace empi {
 
-constexpr int kMaxQ = 8192;
+constexpr int kMaxQ = kMaxSubTransform;   // 4096, longer windows fold onto it
 constexpr int kMinLgQ = 1;    // Q = 2
-constexpr int kMaxLgQ = 13;   // Q = 8192
+constexpr int kMaxLgQ = 12;   // Q = 4096
 constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
 
 // Workgroup geometry of class Q: NSL slots of Q complex points in LDS,
@@ -35,12 +36,11 @@ constexpr int column_waves() { return MODE == kModeSum ? 2 : 1; }
 
 // Tuning variants of the single-channel column kernels: waves per SIMD the
 // register allocator must allow (4, 3, 2), times two persistent grid sizes.
-// Q = 8192 fills a CU's LDS with one workgroup: always 2 waves.
 constexpr int kWaveVariants = 3;
 constexpr int kGridVariants = 2;
 constexpr int kTuneChoices = kWaveVariants * kGridVariants;
 template <int Q, int V>
-constexpr int variant_waves() { return Q >= 8192 ? 4 : (V == 0 ? 4 : V == 1 ? 3 : 2); }
+constexpr int variant_waves() { return V == 0 ? 4 : V == 1 ? 3 : 2; }
 
 typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
                                const UnitDev* units, const UnitState* state, const float* residual,
@@ -48,7 +48,7 @@ typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes,
                                unsigned long long* counter);
 
 // launcher of class lgQ (log2 Q) in mode `mode`; `variant` selects the wave
-// variant of the single-channel kernels.  Q = 2..64 / 128..1024 / 2048..8192.
+// variant of the single-channel kernels.  Q = 2..64 / 128..1024 / 2048..4096.
 ColumnLauncher column_launcher_small(int lgQ, int mode, int variant);
 ColumnLauncher column_launcher_mid(int lgQ, int mode, int variant);
 ColumnLauncher column_launcher_large(int lgQ, int mode, int variant);

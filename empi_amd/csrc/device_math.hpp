@@ -87,7 +87,7 @@ __device__ inline void dft(float2* v) {
 
 // radix of the Stockham pass at sub-length Ns: 8 where the remaining length
 // is a power of 8, otherwise a 16 first so that the passes stay at
-// ceil(log2(left) / 3) or fewer (Q = 8192 = 16 * 8^3, 2048 = 16 * 16 * 8);
+// ceil(log2(left) / 3) or fewer (2048 = 16 * 16 * 8, 1024 = 16 * 8 * 8);
 // 4096 = 16^3 (three passes, measured 2% faster than 8^4)
 constexpr int pick_radix(int left) {
     return left <= 2 ? left : left == 4 ? 4 : (left == 8 || left == 32) ? 8

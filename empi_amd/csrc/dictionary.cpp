@@ -62,7 +62,9 @@ empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictio
         sc.F = static_cast<int>(fCount);
 
         // Longest window over the grid; the packed real transform needs
-        // ceil(W/2) complex inputs, rounded up to a power of two (>= 2).
+        // ceil(W/2) complex inputs, rounded up to a power of two (>= 2) up to
+        // kMaxSubTransform.  Longer inputs fold: the input m = m1 + Q f enters
+        // sub-transform k2 as z[m] e^{-2 pi i k2 f / P}, summed over f.
         int wmax = 0;
         for (int t = 0; t < sc.T; ++t) {
             int iL, iR;
@@ -77,7 +79,10 @@ empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictio
         while (q < (wmax + 1) / 2) q *= 2;
         if (q < 2) q = 2;
         if (q > M) q = M;
+        if (q > kMaxSubTransform) q = kMaxSubTransform;
         sc.Q = q;
+        sc.fold = ((wmax + 1) / 2 + q - 1) / q;
+        if (sc.fold < 1) sc.fold = 1;
         sc.P = M / q;
         sc.G = sc.P >= 2 ? sc.P / 2 : 1;
         out->scales.push_back(sc);

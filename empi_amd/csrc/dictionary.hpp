@@ -16,6 +16,11 @@ namespace empi {
 
 constexpr double kGaussHalfWidth = 3.0;        // src/gabor.cpp:18
 constexpr double kMinScaleInSamples = 2.0;     // src/gabor.cpp:23
+// Largest sub-transform of the column kernels (complex points) and the most
+// Q-point sections a window may fold onto one: windows up to 2 * 4096 * 8 =
+// 65,536 samples.
+constexpr int kMaxSubTransform = 4096;
+constexpr int kMaxFold = 8;
 
 struct Scale {
     double s;        // scale [s]
@@ -26,6 +31,7 @@ struct Scale {
     // launch geometry of the fused column kernels
     int Wmax;        // longest window (samples) over the time grid
     int Q;           // sub-transform length (complex points)
+    int fold;        // Q-point sections of the longest window's packed input
     int P;           // Nfft/2 / Q sub-transforms per column
     int G;           // cells per column = max(1, P/2)
 };

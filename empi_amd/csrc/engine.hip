@@ -1089,9 +1089,9 @@ ColumnLauncher column_launcher(int mode, int cls, int choice) {
          : lg <= 10 ? column_launcher_mid(lg, mode, v) : column_launcher_large(lg, mode, v);
 }
 
-// default tuning: wave variant per class measured on the 64 x 20 x 5120 SMP
-// batch (grid variant 0)
-const int kDefaultTune[kNumClasses] = {0, 0, 2, 1, 0, 0, 2, 1, 1, 1, 1, 0, 0};
+// default tuning: the choice (wave variant + 3 * grid variant) per class that
+// the autotuner picks on the 64 x 20 x 5120 SMP batch
+const int kDefaultTune[kNumClasses] = {0, 0, 3, 5, 3, 3, 3, 3, 3, 3, 3, 3};
 
 int cells_per_wg(int Q) {
     const int nsl = Q >= 2048 ? 2 : std::min(512, 4096 / Q);
@@ -1235,7 +1235,7 @@ empi_status dict_for(empi_plan* p, const empi_dict_params& dp, double fs, int N,
     empi_status st = build_dictionary(dp, fs, N, &d);
     if (st != EMPI_OK) return st;
     for (const Scale& sc : d.scales) {
-        if (sc.Q > kMaxQ) return EMPI_UNSUPPORTED;
+        if (sc.Q > kMaxQ || sc.fold > kMaxFold) return EMPI_UNSUPPORTED;
     }
     *id = static_cast<int>(p->dicts.size());
     p->dicts.push_back(std::move(d));
@@ -1364,6 +1364,7 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             }
             s.gA = static_cast<float>(M_PI / ((sc.s * d.fs) * (sc.s * d.fs)));
             s.Wmax = sc.Wmax;
+            s.fold = sc.fold;
             {
                 const double sfs = sc.s * d.fs;
                 s.csnorm = static_cast<float>(proposal_bound_factor(sc.s, d.fs, sc.Nfft));

@@ -29,6 +29,7 @@ struct ScaleDev {
     float Kp;                 // 2*sqrt(2)/(s*fs)
     float gA;                 // pi/(s*fs)^2
     int Wmax;
+    int fold;                 // Q-point sections of the packed window input (>= 1)
     float invKp;              // 1/K'
     int JT;                   // bins j < JT carry E = exp(-2 pi (j alpha)^2) > 0 in FP32
     const float2* etab;       // [JT]: ((1 - E)/K', 2E/K') in FP32 from FP64
