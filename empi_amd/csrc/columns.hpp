@@ -16,11 +16,18 @@ constexpr int kMinLgQ = 1;    // Q = 2
 constexpr int kMaxLgQ = 12;   // Q = 4096
 constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
 
-// Workgroup geometry of class Q: NSL slots of Q complex points in LDS,
-// 16 points per thread.
+// Workgroup geometry of class Q: NSL slots of Q complex points in LDS (4096
+// points, two slots from Q = 2048 on, at most 512 slots), 16 points per
+// thread.  Shared by the kernels and the host's grid and tile arithmetic.
+constexpr int kClassPoints = 4096;
+constexpr int class_slots(int Q) {
+    return Q >= 2048 ? 2 : (kClassPoints / Q > 512 ? 512 : kClassPoints / Q);
+}
+constexpr int class_threads(int Q) { return class_slots(Q) * Q / 16; }
+
 template <int Q>
 struct Geo {
-    static constexpr int NSL = Q >= 2048 ? 2 : (4096 / Q > 512 ? 512 : 4096 / Q);
+    static constexpr int NSL = class_slots(Q);
     static constexpr int NPTS = NSL * Q;
     static constexpr int EPT = 16;
     static constexpr int NT = NPTS / EPT;

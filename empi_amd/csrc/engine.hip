@@ -1077,7 +1077,7 @@ struct ItemInfo {
 // that with grid variant 1), a multiple of the 8 XCDs, which the kernel's
 // work walk relies on
 int column_grid(int Q, long long tiles, int gv) {
-    const int nt = Q >= 2048 ? (Q * 2) / 16 : 256;
+    const int nt = class_threads(Q);
     const long long cap = (nt >= 1024 ? 512 : nt >= 512 ? 1024 : 2048) << gv;
     const long long g = std::max(1ll, std::min(tiles, cap));
     return static_cast<int>((g + 7) / 8 * 8);
@@ -1093,10 +1093,7 @@ ColumnLauncher column_launcher(int mode, int cls, int choice) {
 // the autotuner picks on the 64 x 20 x 5120 SMP batch
 const int kDefaultTune[kNumClasses] = {0, 0, 3, 5, 3, 3, 3, 3, 3, 3, 3, 3};
 
-int cells_per_wg(int Q) {
-    const int nsl = Q >= 2048 ? 2 : std::min(512, 4096 / Q);
-    return nsl / 2;
-}
+int cells_per_wg(int Q) { return class_slots(Q) / 2; }
 
 }  // namespace
 
