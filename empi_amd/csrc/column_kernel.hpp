@@ -270,7 +270,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     const int m = (tid + (gi * GL + q) * NT) % Q;
                     float2 z = window_pair(ci, m);
                     float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
-                    if (ci.valid) {
+                    // pairs past the window (and every later section) are zero: no twiddles
+                    if (ci.valid && 2 * m < ci.W) {
                         float2 zin_b = z;
                         if constexpr (Q == kMaxQ) {
                             // input m + f Q enters sub-transform k2 with e^{-2 pi i k2 f / P}
