@@ -148,8 +148,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     __shared__ unsigned long long tbest;
     __shared__ unsigned cnt_lds;
     __shared__ float2 etl[kEtabLds + 1];    // ((1 - E)/K', 2E/K') of the item's scale; [kEtabLds] = (1/K', 0)
-    // windows longer than 2Q samples (Q = kMaxQ only): e^{-2 pi i k2 / P} of each cell's two slots
-    __shared__ float2 ftw[Q == kMaxQ ? 2 * NCELL : 1];
+    // windows longer than 2Q samples (Q = 4096 only): e^{-2 pi i k2 / P} of each cell's two slots
+    __shared__ float2 ftw[Q == kMaxSubTransform ? 2 * NCELL : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 ci.dfrac = static_cast<float>(d - dfl);
                 cell_k2(g, P, &ci.ka, &ci.kb);
             }
-            if constexpr (Q == kMaxQ) {
+            if constexpr (Q == kMaxSubTransform) {
                 ftw[2 * c] = cis_rev(-static_cast<float>(ci.ka) / static_cast<float>(P));
                 ftw[2 * c + 1] = ci.kb >= 0 ? cis_rev(-static_cast<float>(ci.kb) / static_cast<float>(P))
                                             : make_float2(0.0f, 0.0f);
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     // pairs past the window (and every later section) are zero: no twiddles
                     if (ci.valid && 2 * m < ci.W) {
                         float2 zin_b = z;
-                        if constexpr (Q == kMaxQ) {
+                        if constexpr (Q == kMaxSubTransform) {
                             // input m + f Q enters sub-transform k2 with e^{-2 pi i k2 f / P}
                             if (sd.fold > 1) {
                                 const float2 ta = ftw[2 * c], tb = ftw[2 * c + 1];
@@ -554,11 +554,18 @@ void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, c
                        state, residual, cellbest, tilebest, counter);
 }
 
+// the launcher of class Q in a multichannel mode
+template <int Q>
+ColumnLauncher multichannel_launcher_for(int mode) {
+    if (mode == kModeSum) return launch_columns<Q, kModeSum, column_waves<Q, kModeSum>()>;
+    if (mode == kModeMmp1) return launch_columns<Q, kModeMmp1, column_waves<Q, kModeMmp1>()>;
+    return nullptr;
+}
+
 // the launcher of class Q in one mode / wave variant
 template <int Q>
 ColumnLauncher launcher_for(int mode, int variant) {
-    if (mode == kModeSum) return launch_columns<Q, kModeSum, column_waves<Q, kModeSum>()>;
-    if (mode == kModeMmp1) return launch_columns<Q, kModeMmp1, column_waves<Q, kModeMmp1>()>;
+    if (mode != kModeSingle) return multichannel_launcher_for<Q>(mode);
     if (variant == 1) return launch_columns<Q, kModeSingle, variant_waves<Q, 1>()>;
     if (variant == 2) return launch_columns<Q, kModeSingle, variant_waves<Q, 2>()>;
     return launch_columns<Q, kModeSingle, variant_waves<Q, 0>()>;

@@ -1,4 +1,4 @@
-// FFT-size classes of the column kernels (Q = 2 .. 4096 complex points per
+// FFT-size classes of the column kernels (Q = 2 .. 8192 complex points per
 // slot), their workgroup geometry, modes and tuning variants, and the launch
 // table the host plan dispatches through.  The kernels are compiled in three
 // translation units by class range; each exports one launcher lookup.
@@ -11,9 +11,9 @@
 
 namespace empi {
 
-constexpr int kMaxQ = kMaxSubTransform;   // 4096, longer windows fold onto it
+constexpr int kMaxQ = kMaxSubTransformMulti;   // 8192: multichannel modes only
 constexpr int kMinLgQ = 1;    // Q = 2
-constexpr int kMaxLgQ = 12;   // Q = 4096
+constexpr int kMaxLgQ = 13;   // Q = 8192
 constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
 
 // Workgroup geometry of class Q: NSL slots of Q complex points in LDS (4096
@@ -55,7 +55,9 @@ typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes,
                                unsigned long long* counter);
 
 // launcher of class lgQ (log2 Q) in mode `mode`; `variant` selects the wave
-// variant of the single-channel kernels.  Q = 2..64 / 128..1024 / 2048..4096.
+// variant of the single-channel kernels; nullptr for a class that has no
+// kernel in that mode (Q = 8192 single-channel).  Q = 2..64 / 128..1024 /
+// 2048..8192.
 ColumnLauncher column_launcher_small(int lgQ, int mode, int variant);
 ColumnLauncher column_launcher_mid(int lgQ, int mode, int variant);
 ColumnLauncher column_launcher_large(int lgQ, int mode, int variant);

@@ -1,4 +1,5 @@
-// Column kernels of the classes Q = 2048 .. 4096 (every mode and wave variant).
+// Column kernels of the classes Q = 2048 .. 8192 (every mode and wave variant;
+// Q = 8192 in the multichannel modes only).
 #include "column_kernel.hpp"
 
 namespace empi {
@@ -7,6 +8,7 @@ ColumnLauncher column_launcher_large(int lgQ, int mode, int variant) {
     switch (lgQ) {
     case 11: return launcher_for<2048>(mode, variant);
     case 12: return launcher_for<4096>(mode, variant);
+    case 13: return multichannel_launcher_for<8192>(mode);
     default: return nullptr;
     }
 }

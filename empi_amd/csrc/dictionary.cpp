@@ -12,8 +12,9 @@ static long pow2_at_least(long x) {   // inc/fftw.hpp:161-168 (minimum 2)
     return y;
 }
 
-empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictionary* out) {
+empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictionary* out, int max_q) {
     *out = Dictionary{};
+    out->max_q = max_q;
     if (!(fs > 0) || N < 2 || !(p.energyError > 0 && p.energyError < 1)) return EMPI_INVALID_ARGUMENT;
     const double energyError = p.energyError;
     const double scaleMin = std::fmax(p.scaleMin, kMinScaleInSamples / fs);
@@ -63,8 +64,9 @@ empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictio
 
         // Longest window over the grid; the packed real transform needs
         // ceil(W/2) complex inputs, rounded up to a power of two (>= 2) up to
-        // kMaxSubTransform.  Longer inputs fold: the input m = m1 + Q f enters
-        // sub-transform k2 as z[m] e^{-2 pi i k2 f / P}, summed over f.
+        // max_q.  Longer inputs fold onto kMaxSubTransform points: the input
+        // m = m1 + Q f enters sub-transform k2 as z[m] e^{-2 pi i k2 f / P},
+        // summed over f.
         int wmax = 0;
         for (int t = 0; t < sc.T; ++t) {
             int iL, iR;
@@ -79,7 +81,7 @@ empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictio
         while (q < (wmax + 1) / 2) q *= 2;
         if (q < 2) q = 2;
         if (q > M) q = M;
-        if (q > kMaxSubTransform) q = kMaxSubTransform;
+        if (q > max_q) q = kMaxSubTransform;
         sc.Q = q;
         sc.fold = ((wmax + 1) / 2 + q - 1) / q;
         if (sc.fold < 1) sc.fold = 1;

@@ -16,10 +16,12 @@ namespace empi {
 
 constexpr double kGaussHalfWidth = 3.0;        // src/gabor.cpp:18
 constexpr double kMinScaleInSamples = 2.0;     // src/gabor.cpp:23
-// Largest sub-transform of the column kernels (complex points) and the most
-// Q-point sections a window may fold onto one: windows up to 2 * 4096 * 8 =
-// 65,536 samples.
+// Sub-transforms of the column kernels (complex points): up to 4096 for
+// single-channel units, 8192 for multichannel ones (their channel loop keeps
+// one workgroup per CU either way).  Longer windows fold onto 4096 points, at
+// most 8 sections: windows up to 2 * 4096 * 8 = 65,536 samples.
 constexpr int kMaxSubTransform = 4096;
+constexpr int kMaxSubTransformMulti = 8192;
 constexpr int kMaxFold = 8;
 
 struct Scale {
@@ -39,6 +41,7 @@ struct Scale {
 struct Dictionary {
     double fs = 0, tMax = 0, root = 0;
     int N = 0;
+    int max_q = kMaxSubTransform;   // longest unfolded sub-transform
     empi_dict_params params{};
     std::vector<Scale> scales;
     long long entries = 0;       // sum F*T
@@ -59,7 +62,10 @@ inline void column_window(double fs, int N, double tMax, int T, double s, int t,
 }
 
 // Builds the dictionary; returns EMPI_OK, EMPI_INVALID_ARGUMENT or EMPI_TOO_LONG.
-empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictionary* out);
+// max_q: the longest sub-transform used unfolded (kMaxSubTransform or
+// kMaxSubTransformMulti); longer inputs fold onto kMaxSubTransform points.
+empi_status build_dictionary(const empi_dict_params& p, double fs, int N, Dictionary* out,
+                             int max_q = kMaxSubTransform);
 
 // Factor c of the proposal's energy bound P <= c * (residual energy under the
 // column's window) for every cell of a scale.  With g the chosen real atom

@@ -1091,7 +1091,7 @@ ColumnLauncher column_launcher(int mode, int cls, int choice) {
 
 // default tuning: the choice (wave variant + 3 * grid variant) per class that
 // the autotuner picks on the 64 x 20 x 5120 SMP batch
-const int kDefaultTune[kNumClasses] = {0, 0, 3, 5, 3, 3, 3, 3, 3, 3, 3, 3};
+const int kDefaultTune[kNumClasses] = {0, 0, 3, 5, 3, 3, 3, 3, 3, 3, 3, 3, 0};
 
 int cells_per_wg(int Q) { return class_slots(Q) / 2; }
 
@@ -1218,10 +1218,11 @@ int lg2(long long x) {
     return l;
 }
 
-empi_status dict_for(empi_plan* p, const empi_dict_params& dp, double fs, int N, int* id) {
+empi_status dict_for(empi_plan* p, const empi_dict_params& dp, double fs, int N, int max_q, int* id) {
     for (size_t i = 0; i < p->dicts.size(); ++i) {
         const Dictionary& d = p->dicts[i];
-        if (d.fs == fs && d.N == N && d.params.energyError == dp.energyError && d.params.scaleMin == dp.scaleMin &&
+        if (d.fs == fs && d.N == N && d.max_q == max_q && d.params.energyError == dp.energyError &&
+            d.params.scaleMin == dp.scaleMin &&
             d.params.scaleMax == dp.scaleMax &&
             (d.params.freqMax == dp.freqMax || (std::isnan(d.params.freqMax) && std::isnan(dp.freqMax)))) {
             *id = static_cast<int>(i);
@@ -1229,7 +1230,7 @@ empi_status dict_for(empi_plan* p, const empi_dict_params& dp, double fs, int N,
         }
     }
     Dictionary d;
-    empi_status st = build_dictionary(dp, fs, N, &d);
+    empi_status st = build_dictionary(dp, fs, N, &d, max_q);
     if (st != EMPI_OK) return st;
     for (const Scale& sc : d.scales) {
         if (sc.Q > kMaxQ || sc.fold > kMaxFold) return EMPI_UNSUPPORTED;
@@ -1252,8 +1253,13 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             return EMPI_INVALID_ARGUMENT;
         const double residualEnergy = 1 - 0.01 * it.energyPercent;
         if (residualEnergy < 0 || residualEnergy >= 1.0) return EMPI_INVALID_ARGUMENT;
+        // multichannel units (MMP1, MMP3 over several channels, constrained
+        // workspaces) keep sub-transforms up to 8192 unfolded
+        const bool multi = (it.variant == EMPI_MMP1 || it.variant == EMPI_MMP3) &&
+                           (it.channels > 1 || it.variant == EMPI_MMP1 || ws_constraint);
         int did = 0;
-        empi_status st = dict_for(p, it.dict, it.fs, it.samples, &did);
+        empi_status st =
+            dict_for(p, it.dict, it.fs, it.samples, multi ? kMaxSubTransformMulti : kMaxSubTransform, &did);
         if (st != EMPI_OK) return st;
         ItemInfo info{};
         info.item = it;
@@ -1406,6 +1412,7 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             const ScaleDev& s = p->scales_h[u.scale0 + l];
             const int cls = s.lgQ - kMinLgQ;
             const int mode = u.constraint ? kModeMmp1 : (u.C > 1 ? kModeSum : kModeSingle);
+            if (!column_launcher(mode, cls, 0)) return EMPI_INTERNAL;   // no kernel for this class and mode
             ClassJobs& cj = p->classes[mode][cls];
             Job j{};
             j.unit = static_cast<int>(ui);
