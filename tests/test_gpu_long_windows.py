@@ -1,17 +1,16 @@
-"""Windows longer than 8192 samples (needs a GPU): the packed input of such a
-window spans several 4096-point sections, which fold onto one sub-transform
-(input m1 + 4096 f enters sub-transform k2 with e^{-2 pi i k2 f / P}).  The
+"""Long windows (needs a GPU).  A single-channel unit's window longer than
+8192 samples spans several 4096-point sections of packed input, which fold
+onto one sub-transform (input m1 + 4096 f enters sub-transform k2 with
+e^{-2 pi i k2 f / P}); multichannel units fold beyond 16,384 samples.  The
 largest scales of a 16,384-sample signal fold twice, of a 32,768-sample
 signal four times; every class runs in the same plan.  Parity against the
 FP64 oracle, as for the short signals."""
-import numpy as np
+import pytest
 
 import empi_amd as E
 import pyoracle as po
 from empi_amd import synth
 from test_gpu_parity import _announce, _check_item, _oracle, _run_plan
-
-import pytest
 
 pytestmark = pytest.mark.gpu
 
@@ -26,3 +25,18 @@ def test_folded_windows_match_oracle(N, iters):
     assert res.status == 0
     ora = _oracle(x[None, :], 256.0, 0.01, po.SMP, iters, 99.0)
     _check_item(res, ora, E.SMP, f"{N} samples")
+
+
+@pytest.mark.parametrize("variant", [E.MMP3, E.MMP1])
+def test_folded_windows_multichannel(variant):
+    """Two channels of 20,000 samples: multichannel units run windows up to
+    16,384 samples unfolded (8192-point sub-transforms) and fold the longer
+    ones three times onto 4096 points, in the sum and MMP1 modes."""
+    _announce(f"long windows, 2 channels x 20000 samples @256 Hz, variant {variant}, 3 iterations")
+    x = synth.shared_multichannel(C=2, N=20000, fs=256.0, atoms=4, noise_sigma=0.1, seed=4400)
+    spec = E.ItemSpec(2, 20000, 256.0, variant, 3, 99.0, 0.01)
+    plan = _run_plan([spec], [x])
+    res = plan.result(0)
+    assert res.status == 0
+    ora = _oracle(x, 256.0, 0.01, variant, 3, 99.0)
+    _check_item(res, ora, variant, f"2 x 20000 samples, variant {variant}")
