@@ -207,6 +207,20 @@ def measured_traffic(kernel_q, desc):
     return None, None
 
 
+def measured_valu(kernel_q, desc):
+    """VALU lane-instructions per map entry of the dominant column class from
+    the committed SQ_INSTS_VALU summary of the same workload, else None."""
+    path = os.path.join(ROOT, "profiles", "k_columns_valu_mix_smp_1280x5120.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    per = d["valu_lane_instr_per_entry"].get(str(kernel_q))
+    if d.get("workload") != desc or per is None:
+        return None
+    return per, d["peak_wave_instr_per_s"], os.path.relpath(path, ROOT)
+
+
 def copy_calibration():
     """One device-to-device copy of exactly 1 GiB: the reference reading for
     the FETCH_SIZE / WRITE_SIZE counters (expected 1048576 KiB each)."""
@@ -298,6 +312,14 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": f"k_columns<{dom['sub_length']}>", "avg_launch_ms": avg_ms,
                 "entries_per_launch": dom["entries"] / dom["launches"]}
+    # the bound that binds: VALU issue (wave instructions per second against
+    # 256 CUs x 4 SIMDs x one wave64 instruction per 4 cycles)
+    valu = measured_valu(dom["sub_length"], desc)
+    if valu:
+        per_entry, peak, src = valu
+        rate = roofline["entries_per_launch"] * per_entry / 64 / (avg_ms * 1e-3)
+        roofline["valu"] = {"lane_instr_per_entry": per_entry, "wave_instr_per_s": rate,
+                            "peak_wave_instr_per_s": peak, "issue_frac": rate / peak, "source": src}
 
     prop = None
     if args.workload == "smp-batch" and not args.no_proposal:
