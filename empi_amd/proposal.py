@@ -117,15 +117,85 @@ def training_set(batches=(5000, 6000), items=512, iterations=100, stride=64, cap
     return np.concatenate(Xs), np.concatenate(ts), total
 
 
+def _explained(plan, n):
+    """(explained fraction per signal, explained energy per signal)."""
+    frac, energy = [], []
+    for i in range(n):
+        e = plan.result(i).energies[0]
+        e = e[~np.isnan(e)]
+        frac.append(1.0 - e[-1] / e[0])
+        energy.append(e[0] - e[-1])
+    return np.array(frac), np.array(energy)
+
+
+def sweep(deltas, items=512, top_k=8, log=print):
+    """Atoms/s and explained-energy deficit of the proposal against the plain
+    method over a range of delta (the per-iteration tolerance: every chosen
+    atom's product is at least (1 - delta) times the exact maximum wherever
+    the bounds hold), on the uneven benchmark batch (512 signals of
+    1024..16384 samples at 256 Hz, seeds 4000+i).  Each run is timed after an
+    untimed one.  Deficits per signal (max, mean) and of the batch (explained
+    energy summed over signals)."""
+    import time
+    import torch
+    from . import ItemSpec, PROPOSAL_ON, SMP
+    from .synth import uneven_batch
+    xs = uneven_batch(items=items)
+    specs = [ItemSpec(1, len(x), 256.0, SMP, 100, 99.0, 0.01) for x in xs]
+    dev = torch.from_numpy(np.concatenate([x.astype(np.float32) for x in xs])).to("cuda:0")
+    net = load()
+    tuning = None
+
+    def timed(delta):
+        nonlocal tuning
+        plan = Plan(specs)
+        if tuning is None:
+            tuning = plan.autotune(device_ptr=dev.data_ptr(), repeats=2)
+        else:
+            plan.set_tuning(tuning)
+        if delta is not None:
+            plan.set_proposal(PROPOSAL_ON, net, top_k=top_k, delta=delta)
+        plan.run(device_ptr=dev.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run(device_ptr=dev.data_ptr())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        atoms = sum(int(plan.result(i).counts.sum()) for i in range(len(specs)))
+        frac, energy = _explained(plan, len(specs))
+        entries = plan.entries_evaluated()
+        plan.close()
+        return atoms / dt, entries, frac, energy
+
+    base_rate, base_entries, base_frac, base_energy = timed(None)
+    log(f"plain: {base_rate:.0f} atoms/s")
+    rows = []
+    for delta in deltas:
+        rate, entries, frac, energy = timed(delta)
+        d = (base_frac - frac) / base_frac
+        row = {"delta": delta, "atoms_per_s": rate, "speedup": rate / base_rate,
+               "entries_ratio": base_entries / entries, "deficit_max": float(d.max()),
+               "deficit_mean": float(d.mean()),
+               "deficit_batch": float((base_energy.sum() - energy.sum()) / base_energy.sum())}
+        log(row)
+        rows.append(row)
+    return base_rate, rows
+
+
 def main():
     import argparse
     import time
-    ap = argparse.ArgumentParser(description="train the proposal network on the device")
+    ap = argparse.ArgumentParser(description="train the proposal network on the device, or sweep delta")
     ap.add_argument("--items", type=int, default=512)
     ap.add_argument("--stride", type=int, default=64)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--sweep", default=None,
+                    help="comma-separated deltas: measure speed and deficit against the plain method instead of training")
     a = ap.parse_args()
+    if a.sweep:
+        sweep([float(v) for v in a.sweep.split(",")], items=a.items, log=lambda m: print(m, flush=True))
+        return
     t0 = time.perf_counter()
     X, t, atoms = training_set(items=a.items, stride=a.stride, log=lambda m: print(m, flush=True))
     print(f"collected {len(t)} rows from {atoms} atoms in {time.perf_counter() - t0:.1f} s", flush=True)
