@@ -119,9 +119,8 @@ struct CellInfo {
     int W;
     int dint;
     float dfrac;
-    int cell;
     int ka, kb;     // sub-transforms of the cell's two slots, -1 = none
-};
+};   // 32 bytes: the Q = 32 class fits four workgroups per CU
 
 // MODE kModeSingle: one channel, no constraint (SMP units, MMP2 sums);
 // kModeSum: sum over channels of the products (MMP3); kModeMmp1: MMP1
@@ -186,13 +185,13 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
 
         for (int c = tid; c < NCELL; c += NT) {
             CellInfo ci{};
-            ci.cell = cell0 + c;
-            ci.exists = ci.cell < ncells;
+            const int cell = cell0 + c;
+            ci.exists = cell < ncells;
             ci.valid = ci.exists;
             ci.kb = -1;
             if (ci.exists) {
-                const int t = ci.cell / G;
-                const int g = ci.cell - t * G;
+                const int t = cell / G;
+                const int g = cell - t * G;
                 const double t0 = t * sd.tMax / (sd.T - 1);
                 ci.valid = t >= item.tlo && t <= item.thi;
                 if (item.atom) ci.valid = ci.valid && fabs(t0 - st.tA) <= (s + st.sA) * kGaussHalfWidth;
@@ -522,9 +521,9 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             unsigned long long cb;
             if (ci.valid) {
                 cb = best[c];
-                cellbest[job.cell_off + ci.cell] = cb;
+                cellbest[job.cell_off + cell0 + c] = cb;
             } else {
-                cb = cellbest[job.cell_off + ci.cell];
+                cb = cellbest[job.cell_off + cell0 + c];
             }
             const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(cb);
             const unsigned key = (bin << 10) | static_cast<unsigned>(c);
