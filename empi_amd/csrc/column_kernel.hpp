@@ -18,12 +18,18 @@ namespace empi {
 // E-table entries staged in LDS per work item (scales with more fall back to HBM)
 constexpr int kEtabLds = 128;
 
-// LDS index with one float2 of padding per 8: the stride-8 writes of the
-// first radix-8 passes then spread over all banks.
-__device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> 3); }
-template <int NPTS>
+// LDS index with one float2 of padding per 8 points: the strided writes of
+// the radix-16/8 passes then spread over all banks.  Classes Q <= 16 run one
+// pass whose lanes each read 16 consecutive points: one float2 per 16 puts
+// neighbouring lanes 17 points apart (and saves the LDS that lets Q = 16
+// hold four workgroups per CU).
+template <int Q>
+constexpr int pad_shift() { return Q <= 16 ? 4 : 3; }
+template <int Q>
+__device__ __forceinline__ constexpr int lpad(int i) { return i + (i >> pad_shift<Q>()); }
+template <int Q, int NPTS>
 struct Lds {
-    static constexpr int SIZE = NPTS + NPTS / 8;
+    static constexpr int SIZE = NPTS + (NPTS >> pad_shift<Q>());
 };
 
 // One Stockham autosort pass of radix R at sub-length Ns over all slots.
@@ -40,7 +46,7 @@ __device__ inline void stockham_pass(float2* buf) {
         const int bi = tid + b * NT;
         const int slot = bi / BPS, j = bi % BPS;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = buf[lpad(slot * Q + j + r * BPS)];
+        for (int r = 0; r < R; ++r) v[b][r] = buf[lpad<Q>(slot * Q + j + r * BPS)];
     }
     __syncthreads();
 #pragma unroll
@@ -75,15 +81,16 @@ __device__ inline void stockham_pass(float2* buf) {
         }
         dft<R>(v[b]);
         const int base = slot * Q + (j - k) * R + k;
-        if constexpr (Ns % 8 == 0) {
-            // 8 | r Ns: lpad(base + r Ns) = lpad(base) + r (Ns + Ns / 8), one
-            // address and constant offsets
-            const int pb = lpad(base);
+        constexpr int SH = pad_shift<Q>();
+        if constexpr (Ns % (1 << SH) == 0) {
+            // 2^SH | r Ns: lpad(base + r Ns) = lpad(base) + r (Ns + Ns / 2^SH),
+            // one address and constant offsets
+            const int pb = lpad<Q>(base);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[pb + r * (Ns + Ns / 8)] = v[b][r];
+            for (int r = 0; r < R; ++r) buf[pb + r * (Ns + (Ns >> SH))] = v[b][r];
         } else {
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[lpad(base + r * Ns)] = v[b][r];
+            for (int r = 0; r < R; ++r) buf[lpad<Q>(base + r * Ns)] = v[b][r];
         }
     }
     __syncthreads();
@@ -113,14 +120,14 @@ __device__ inline void cell_k2(int g, int P, int* ka, int* kb) {
 // ------------------------------------------------------------ column kernel
 
 struct CellInfo {
-    int valid;      // re-evaluated by this pass
-    int exists;     // cell < T*G
+    unsigned short valid;    // re-evaluated by this pass
+    unsigned short exists;   // cell < T*G
     int iL;
     int W;
     int dint;
     float dfrac;
     int ka, kb;     // sub-transforms of the cell's two slots, -1 = none
-};   // 32 bytes: the Q = 32 class fits four workgroups per CU
+};   // 28 bytes: the Q = 16 and 32 classes fit four workgroups per CU
 
 // MODE kModeSingle: one channel, no constraint (SMP units, MMP2 sums);
 // kModeSum: sum over channels of the products (MMP3); kModeMmp1: MMP1
@@ -141,7 +148,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     constexpr int SEG = 2 * Q < 64 ? 2 * Q : 64;          // lanes sharing a cell
     constexpr int GRP = 2 * Q >= NT ? 2 * Q / NT : 1;      // consecutive elements sharing a cell
     static_assert(EPT % GRP == 0, "element groups");
-    __shared__ float2 buf[Lds<NPTS>::SIZE];
+    __shared__ float2 buf[Lds<Q, NPTS>::SIZE];
     __shared__ CellInfo info[NCELL];
     __shared__ unsigned long long best[NCELL];
     __shared__ unsigned long long tbest;
@@ -297,8 +304,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                         if (ci.kb >= 0)
                             zb = cmul(zin_b, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.kb), um)) * invM));
                     }
-                    buf[lpad((2 * c) * Q + m)] = za;
-                    buf[lpad((2 * c + 1) * Q + m)] = zb;
+                    buf[lpad<Q>((2 * c) * Q + m)] = za;
+                    buf[lpad<Q>((2 * c + 1) * Q + m)] = zb;
                 }
             }
             __syncthreads();
@@ -356,10 +363,10 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
                 if (!(ci.valid && k2 >= 0 && k < F)) return;
                 const int slot = 2 * c + half;
-                const float2 Zk = buf[lpad(slot * Q + k1)];
+                const float2 Zk = buf[lpad<Q>(slot * Q + k1)];
                 float2 Zp;
-                if (k2 == 0) Zp = buf[lpad(slot * Q + ((Q - k1) & (Q - 1)))];
-                else Zp = buf[lpad(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
+                if (k2 == 0) Zp = buf[lpad<Q>(slot * Q + ((Q - k1) & (Q - 1)))];
+                else Zp = buf[lpad<Q>(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
                 const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
                 const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
                 const float2 wv = cis_rev(-static_cast<float>(k) * invN);
@@ -387,8 +394,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 }
                 const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(ka1))) + k2;
                 const int km = M - k;
-                const float2 Zk = buf[lpad(sa * Q + ka1)];
-                const float2 Zp = buf[lpad(sb * Q + kb1)];
+                const float2 Zk = buf[lpad<Q>(sa * Q + ka1)];
+                const float2 Zp = buf[lpad<Q>(sb * Q + kb1)];
                 const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
                 const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
                 const float2 wv = cis_rev(-static_cast<float>(k) * invN);
@@ -397,7 +404,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 if (km < F && km != M) from_x(km, 0.5f * (Ar - pi), -0.5f * (Ai + pr), ci, 0, run);
                 if (ci.ka == 0 && u == 0 && M / 2 < F) {
                     // (a, Q/2): bin M/2, its own mirror (Z[k] = Z[M-k])
-                    const float2 Zh = buf[lpad(2 * c * Q + Q / 2)];
+                    const float2 Zh = buf[lpad<Q>(2 * c * Q + Q / 2)];
                     const float2 wh = cis_rev(-static_cast<float>(M / 2) * invN);
                     const float hBi = Zh.y + Zh.y;                 // B = (0, 2 Im Zh), A = (2 Re Zh, 0)
                     const float hpr = -wh.y * hBi, hpi = wh.x * hBi;
@@ -450,7 +457,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             if (tid < NCELL && F > M) {
                 const CellInfo& ci = info[tid];
                 if (ci.valid && ci.ka == 0) {
-                    const float2 Z0 = buf[lpad((2 * tid) * Q)];
+                    const float2 Z0 = buf[lpad<Q>((2 * tid) * Q)];
                     const float a = Z0.x - Z0.y;
                     accNyq += a * a;
                 }
