@@ -120,8 +120,7 @@ __device__ inline void cell_k2(int g, int P, int* ka, int* kb) {
 // ------------------------------------------------------------ column kernel
 
 struct CellInfo {
-    unsigned short valid;    // re-evaluated by this pass
-    unsigned short exists;   // cell < T*G
+    int valid;      // re-evaluated by this pass (cells past T*G never are)
     int iL;
     int W;
     int dint;
@@ -193,10 +192,10 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         for (int c = tid; c < NCELL; c += NT) {
             CellInfo ci{};
             const int cell = cell0 + c;
-            ci.exists = cell < ncells;
-            ci.valid = ci.exists;
+            const bool exists = cell < ncells;
+            ci.valid = exists;
             ci.kb = -1;
-            if (ci.exists) {
+            if (exists) {
                 const int t = cell / G;
                 const int g = cell - t * G;
                 const double t0 = t * sd.tMax / (sd.T - 1);
@@ -524,7 +523,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         unsigned long long tb = 0ull;
         for (int c = tid; c < NCELL; c += NT) {
             const CellInfo& ci = info[c];
-            if (!ci.exists) continue;
+            if (cell0 + c >= ncells) continue;
             unsigned long long cb;
             if (ci.valid) {
                 cb = best[c];
