@@ -60,7 +60,7 @@ def build_host():
     if len(srcs) != len(HOST_SOURCES):
         return None
     tmp = CLI + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-pthread", "-I" + os.path.join(ROOT, "include"),
           "-I" + os.path.join(HERE, "host"), "-o", tmp] + srcs +
          ["-L" + HERE, "-lempi_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, CLI)

@@ -8,8 +8,13 @@
 // with '-' are ignored for MP5 compatibility, except:
 //   --per-call   run the reference loop on the host over the per-call
 //                Workspace ABI (one device round trip per atom)
+//   --epochs-per-batch=N   decompose N epochs per device batch (default: all)
 // By default every epoch of the file is decomposed together in one
-// device-resident batch and the progress lines are printed afterwards.
+// device-resident batch and the progress lines are printed afterwards.  With
+// --epochs-per-batch the file is streamed: a reader thread reads the next N
+// epochs while the device decomposes the current ones, the batch plan is
+// built once and reused, and each batch's lines and book records are
+// written, in epoch order, as soon as it is done.
 #include <algorithm>
 #include <cctype>
 #include <climits>
@@ -20,7 +25,9 @@
 #include <iostream>
 #include <memory>
 #include <new>
+#include <exception>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "book.hpp"
@@ -143,23 +150,13 @@ void run_per_call(Run& r) {
     writer.close();
 }
 
-// every epoch in one device-resident batch; output replayed in reference order
-void run_batched(Run& r) {
-    BookWriter writer(r.bookPath);
-    std::vector<Signal> epochs;
-    while (true) {
-        Signal s = r.reader->read();
-        if (s.samples == 0) break;
-        // one workspace sized from the first epoch serves every epoch (src/empi.cpp:154-158)
-        if (!epochs.empty() && s.samples != epochs.front().samples) throw Error("internalLogicError");
-        epochs.push_back(std::move(s));
-    }
-    BatchDecomposition batch;
-    std::vector<BatchDecomposition::EpochResult> results =
-        batch.run(epochs, r.variant, r.settings, r.dict);
+// Progress lines and book records of one batch, epochs numbered from first + 1.
+void write_batch(const Run& r, BookWriter& writer, const std::vector<Signal>& epochs,
+                 const std::vector<BatchDecomposition::EpochResult>& results, int first) {
     for (size_t e = 0; e < epochs.size(); ++e) {
         const BatchDecomposition::EpochResult& res = results[e];
-        start_line(static_cast<int>(e) + 1, r);
+        const int epoch = first + static_cast<int>(e) + 1;
+        start_line(epoch, r);
         const bool failed = res.status == EMPI_ENERGY_MAP_IS_EMPTY;
         if (r.variant == EMPI_SMP) {
             for (int c = 0; c < epochs[e].channels; ++c) {
@@ -180,8 +177,57 @@ void run_batched(Run& r) {
             if (failed) throw Error("energyMapIsEmpty");
         }
         if (res.status != EMPI_OK) throw Error(empi_strerror(res.status));
-        writer.write(static_cast<int>(e) + 1, epochs[e], res.atoms);
+        writer.write(epoch, epochs[e], res.atoms);
     }
+}
+
+// Epochs in device-resident batches of per_batch (0: all of them); output
+// replayed in reference order.  While a batch runs on the device, a reader
+// thread reads the next one.  A read error ends the stream after the epochs
+// read before it are decomposed and written, as in the reference's
+// epoch-by-epoch loop.
+void run_batched(Run& r, int per_batch) {
+    BookWriter writer(r.bookPath);
+    int samples = -1;
+    auto read_batch = [&](std::vector<Signal>& out, std::exception_ptr& error) {
+        out.clear();
+        try {
+            while (per_batch <= 0 || static_cast<int>(out.size()) < per_batch) {
+                Signal s = r.reader->read();
+                if (s.samples == 0) break;
+                // one workspace sized from the first epoch serves every epoch (src/empi.cpp:154-158)
+                if (samples >= 0 && s.samples != samples) throw Error("internalLogicError");
+                samples = s.samples;
+                out.push_back(std::move(s));
+            }
+        } catch (...) {
+            error = std::current_exception();
+        }
+    };
+    BatchDecomposition batch;
+    std::vector<Signal> epochs, next;
+    std::exception_ptr error;
+    read_batch(epochs, error);
+    int first = 0;   // epochs written so far
+    while (!epochs.empty()) {
+        std::exception_ptr next_error;
+        std::thread reader;
+        if (!error) reader = std::thread([&] { read_batch(next, next_error); });
+        std::vector<BatchDecomposition::EpochResult> results;
+        try {
+            results = batch.run(epochs, r.variant, r.settings, r.dict);
+        } catch (...) {
+            if (reader.joinable()) reader.join();
+            throw;
+        }
+        if (reader.joinable()) reader.join();
+        write_batch(r, writer, epochs, results, first);
+        first += static_cast<int>(epochs.size());
+        if (error) break;
+        epochs.swap(next);
+        error = next_error;
+    }
+    if (error) std::rethrow_exception(error);
     writer.close();
 }
 
@@ -195,18 +241,26 @@ void fail(const char* message) {
 int main(int argc, char** argv) {
     const char* config = nullptr;
     bool per_call = false;
+    int per_batch = 0;
     for (int i = 1; i < argc; ++i) {
         if (std::strcmp(argv[i], "--per-call") == 0) per_call = true;
+        if (std::strncmp(argv[i], "--epochs-per-batch=", 19) == 0) {
+            per_batch = std::atoi(argv[i] + 19);
+            if (per_batch <= 0) {
+                std::fprintf(stderr, "%s: --epochs-per-batch needs a positive count\n", argv[0]);
+                return 1;
+            }
+        }
         if (*argv[i] != '-' && *argv[i] != 0) config = argv[i];
     }
     if (!config) {
-        std::fprintf(stderr, "USAGE: %s [--per-call] path_to_config_file\n", argv[0]);
+        std::fprintf(stderr, "USAGE: %s [--per-call] [--epochs-per-batch=N] path_to_config_file\n", argv[0]);
         return 1;
     }
     try {
         Run r = configure(config);
         if (per_call) run_per_call(r);
-        else run_batched(r);
+        else run_batched(r, per_batch);
         std::puts(" END");
     } catch (const Error& e) {
         fail(e.what());

@@ -162,9 +162,22 @@ std::vector<BatchDecomposition::EpochResult> BatchDecomposition::run(const std::
     input.reserve(total);
     for (const Signal& s : epochs)
         for (double v : s.data) input.push_back(static_cast<float>(v));
-    empi_plan* plan = nullptr;
-    check(empi_plan_create(items.data(), static_cast<int>(items.size()), &plan));
-    std::unique_ptr<empi_plan, void (*)(empi_plan*)> guard(plan, empi_plan_destroy);
+    auto same_item = [](const empi_item& a, const empi_item& b) {
+        return a.channels == b.channels && a.samples == b.samples && a.fs == b.fs && a.variant == b.variant &&
+               a.iterationMax == b.iterationMax && a.energyPercent == b.energyPercent &&
+               a.dict.energyError == b.dict.energyError && a.dict.scaleMin == b.dict.scaleMin &&
+               a.dict.scaleMax == b.dict.scaleMax && a.dict.freqMax == b.dict.freqMax;
+    };
+    const bool same = items.size() == items_.size() && std::equal(items.begin(), items.end(), items_.begin(), same_item);
+    if (!same) {
+        plan_.reset();
+        items_.clear();
+        empi_plan* created = nullptr;
+        check(empi_plan_create(items.data(), static_cast<int>(items.size()), &created));
+        plan_.reset(created);
+        items_ = items;
+    }
+    empi_plan* plan = plan_.get();
     check(empi_plan_run(plan, input.data(), 0, nullptr));
     const int I = settings.iterationMax;
     for (size_t e = 0; e < epochs.size(); ++e) {

@@ -105,12 +105,20 @@ public:
         std::vector<int> counts;
         size_t atomCount = 0;
     };
+    // Decomposes the epochs as one device batch.  The plan (dictionaries,
+    // device buffers, captured graph) is kept and reused by the next call
+    // with the same epoch count and shapes, so a stream of equal chunks
+    // builds it once.
     std::vector<EpochResult> run(const std::vector<Signal>& epochs, int variant,
                                  const DecompositionSettings& settings, const empi_dict_params& dict);
     // ATOM lines of one progress row (src/classes.cpp:41-48), the failing
     // search's line included when status is energyMapIsEmpty.
     static void print_progress(std::ostream& out, const std::vector<double>& energies, int count, size_t atomCount,
                                int iterationMax, bool failed_search);
+
+private:
+    std::unique_ptr<empi_plan, void (*)(empi_plan*)> plan_{nullptr, empi_plan_destroy};
+    std::vector<empi_item> items_;   // the items plan_ was built for
 };
 
 }  // namespace host

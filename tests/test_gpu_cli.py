@@ -105,3 +105,32 @@ def test_cli_mmp_variants_name_and_run(tmp_path):
     for c in range(4):
         for i in range(m):
             assert np.allclose(atoms[c * n + i][1][2:5], ora.params[c, i, 2:5].astype(np.float32), rtol=1e-6)
+
+
+@pytest.mark.parametrize("per_batch", [2, 3])
+def test_cli_streamed_batches_match_one_batch(tmp_path, per_batch):
+    """--epochs-per-batch streams the file in device batches (reader thread,
+    reused plan, the last batch shorter): stdout and the book are byte-for-byte
+    those of the single batch, for SMP and MMP3."""
+    print(f"\n[gpu case] CLI 5 epochs, one batch vs --epochs-per-batch={per_batch}", flush=True)
+    N = 1024
+    for mp, C in [("SMP", 2), ("MMP3", 3)]:
+        x = np.concatenate([synth.shared_multichannel(C=C, N=N, fs=256.0, atoms=3, seed=50 + e) for e in range(5)],
+                           axis=1)                           # [channel][5N]
+        case = tmp_path / f"{mp}_{per_batch}"
+        case.mkdir()
+        cfg = _write_case(case, x.T.copy(),
+                          f"energyError 0.01\nmaximalNumberOfIterations 12\nenergyPercent 99\nMP {mp}\n"
+                          f"samplingFrequency 256\nnumberOfChannels {C}\nselectedChannels 1-{C}\n"
+                          f"numberOfSamplesInEpoch {N}\n")
+        book = case / f"sig_{mp[:3].lower()}.b"
+        one = subprocess.run([CLI, str(cfg)], capture_output=True, text=True, timeout=300)
+        assert one.returncode == 0, one.stdout[-2000:]
+        one_book = book.read_bytes()
+        book.unlink()
+        streamed = subprocess.run([CLI, f"--epochs-per-batch={per_batch}", str(cfg)], capture_output=True,
+                                  text=True, timeout=300)
+        assert streamed.returncode == 0, streamed.stdout[-2000:]
+        assert streamed.stdout == one.stdout
+        assert book.read_bytes() == one_book
+        assert one.stdout.count("START\t") == 5
