@@ -839,7 +839,10 @@ __global__ void k_reset_counts(const ClassDev* __restrict__ classes, int n, int*
     if (i == 0 && extra) *extra = 0;
 }
 
-// work items of the regions flagged "evaluate" (one workgroup per unit)
+// One work item per flagged region of unit blockIdx.x, appended to its
+// class's list.  The items of each class are counted in LDS first and their
+// range reserved with one global atomic per class and workgroup: per-region
+// atomics on a dozen class counters serialise in L2.
 __global__ __launch_bounds__(kRegionThreads) void k_region_items(const ClassDev* __restrict__ classes,
                                                                  const UnitDev* __restrict__ units,
                                                                  const UnitState* __restrict__ state,
@@ -848,12 +851,36 @@ __global__ __launch_bounds__(kRegionThreads) void k_region_items(const ClassDev*
                                                                  const RegionDev* __restrict__ regtab,
                                                                  const RegionState* __restrict__ regions,
                                                                  unsigned long long* __restrict__ stat) {
+    constexpr int kSlots = 3 * kNumClasses;
+    __shared__ int lcount[kSlots];
+    __shared__ int lbase[kSlots];
+    __shared__ unsigned nflag;
     const int u = blockIdx.x;
-    if (!state[u].active) return;
+    if (!state[u].active) return;   // uniform over the workgroup
     const UnitDev ud = units[u];
-    for (int r = threadIdx.x; r < ud.L * ud.B; r += kRegionThreads) {
+    const int tid = threadIdx.x;
+    const int nreg = ud.L * ud.B;
+    for (int i = tid; i < kSlots; i += kRegionThreads) lcount[i] = 0;
+    if (tid == 0) nflag = 0;
+    __syncthreads();
+    unsigned flagged = 0;
+    for (int r = tid; r < nreg; r += kRegionThreads) {
         if (!(regions[ud.reg_off + r].flags & 2)) continue;
-        if (stat) atomicAdd(stat, 1ull);
+        ++flagged;
+        const RegionDev rt = regtab[ud.regtab_off + r];
+        if (rt.tlo <= rt.thi) atomicAdd(&lcount[uscales[ud.us_off + r / ud.B].cls_slot], 1);
+    }
+    if (flagged) atomicAdd(&nflag, flagged);
+    __syncthreads();
+    // slots with items are ClassDev entries of this plan
+    for (int i = tid; i < kSlots; i += kRegionThreads) {
+        lbase[i] = lcount[i] ? atomicAdd(classes[i].count, lcount[i]) : 0;
+        lcount[i] = 0;
+    }
+    if (tid == 0 && stat && nflag) atomicAdd(stat, static_cast<unsigned long long>(nflag));
+    __syncthreads();
+    for (int r = tid; r < nreg; r += kRegionThreads) {
+        if (!(regions[ud.reg_off + r].flags & 2)) continue;
         const RegionDev rt = regtab[ud.regtab_off + r];
         if (rt.tlo > rt.thi) continue;
         const int l = r / ud.B;
@@ -868,7 +895,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_region_items(const ClassDev*
         it.thi = rt.thi;
         it.atom = 0;
         it.region = static_cast<int>(ud.reg_off + r);
-        const int pos = atomicAdd(cd.count, 1);
+        const int pos = lbase[us.cls_slot] + atomicAdd(&lcount[us.cls_slot], 1);
         if (pos < cd.max_items) cd.items[pos] = it;
     }
 }

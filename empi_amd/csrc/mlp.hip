@@ -104,6 +104,7 @@ __global__ __launch_bounds__(NT) void k_mlp_forward(const float* __restrict__ X,
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
 constexpr int kMfmaRows = 32;
+constexpr int kMfmaMaxBlocks = 2048;   // 8 per CU
 constexpr int kMaxDin = 256;
 
 __global__ __launch_bounds__(NT) void k_mlp_forward_mfma(const float* __restrict__ X, int R,
@@ -116,73 +117,76 @@ __global__ __launch_bounds__(NT) void k_mlp_forward_mfma(const float* __restrict
     __shared__ _Float16 h1[kMfmaRows][H + 8];
     __shared__ float red[NT / 64][kMfmaRows];
     const int tid = threadIdx.x;
-    const int r0 = blockIdx.x * kMfmaRows;
     if (Rdev) R = min(R, *Rdev);
-    if (r0 >= R) return;   // uniform over the workgroup
     const int kp = (din + 7) / 8 * 8;
-    for (int i = tid; i < kMfmaRows * kp; i += NT) {
-        const int r = i / kp, c = i % kp;
-        xs[r][c] = static_cast<_Float16>(r0 + r < R && c < din ? X[static_cast<long long>(r0 + r) * din + c] : 0.0f);
-    }
-    __syncthreads();
-    const int wave = tid >> 6, l = tid & 63;
-    const int lr = l % 32, lk = 4 * (l / 32);
-    const __half_raw* W1r = reinterpret_cast<const __half_raw*>(W1);
-    const __half_raw* W2r = reinterpret_cast<const __half_raw*>(W2);
-    auto hw = [](const __half_raw* w, long long i) {
-        _Float16 v;
-        __builtin_memcpy(&v, &w[i].data, sizeof v);
-        return v;
-    };
-    // layer 1
-    for (int nt = 0; nt < 2; ++nt) {
-        const int col = 64 * wave + 32 * nt + lr;
-        float16_t acc;
-        for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
-        for (int k0 = 0; k0 < kp; k0 += 8) {
-            half4_t a, b;
-            for (int k = 0; k < 4; ++k) {
-                a[k] = xs[lr][k0 + lk + k];
-                const int kk = k0 + lk + k;
-                b[k] = kk < din ? hw(W1r, static_cast<long long>(col) * din + kk) : static_cast<_Float16>(0.0f);
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
+    // a bounded grid strides over the 32-row tiles: the launch is sized for
+    // the most rows a pass can have, the device count is usually far lower
+    for (int r0 = blockIdx.x * kMfmaRows; r0 < R; r0 += gridDim.x * kMfmaRows) {   // uniform per workgroup
+        for (int i = tid; i < kMfmaRows * kp; i += NT) {
+            const int r = i / kp, c = i % kp;
+            xs[r][c] = static_cast<_Float16>(r0 + r < R && c < din ? X[static_cast<long long>(r0 + r) * din + c] : 0.0f);
         }
-        const float bias = b1[col];
-        for (int j = 0; j < 16; ++j) h1[8 * (j / 4) + lk + (j % 4)][col] = static_cast<_Float16>(fmaxf(acc[j] + bias, 0.0f));
-    }
-    __syncthreads();
-    // layer 2 and the output dot product
-    float part[16];
-    for (int j = 0; j < 16; ++j) part[j] = 0.0f;
-    for (int nt = 0; nt < 2; ++nt) {
-        const int col = 64 * wave + 32 * nt + lr;
-        float16_t acc;
-        for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
-        for (int k0 = 0; k0 < H; k0 += 8) {
-            half4_t a, b;
-            for (int k = 0; k < 4; ++k) {
-                a[k] = h1[lr][k0 + lk + k];
-                b[k] = hw(W2r, static_cast<long long>(col) * H + k0 + lk + k);
+        __syncthreads();
+        const int wave = tid >> 6, l = tid & 63;
+        const int lr = l % 32, lk = 4 * (l / 32);
+        const __half_raw* W1r = reinterpret_cast<const __half_raw*>(W1);
+        const __half_raw* W2r = reinterpret_cast<const __half_raw*>(W2);
+        auto hw = [](const __half_raw* w, long long i) {
+            _Float16 v;
+            __builtin_memcpy(&v, &w[i].data, sizeof v);
+            return v;
+        };
+        // layer 1
+        for (int nt = 0; nt < 2; ++nt) {
+            const int col = 64 * wave + 32 * nt + lr;
+            float16_t acc;
+            for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+            for (int k0 = 0; k0 < kp; k0 += 8) {
+                half4_t a, b;
+                for (int k = 0; k < 4; ++k) {
+                    a[k] = xs[lr][k0 + lk + k];
+                    const int kk = k0 + lk + k;
+                    b[k] = kk < din ? hw(W1r, static_cast<long long>(col) * din + kk) : static_cast<_Float16>(0.0f);
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
             }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
+            const float bias = b1[col];
+            for (int j = 0; j < 16; ++j) h1[8 * (j / 4) + lk + (j % 4)][col] = static_cast<_Float16>(fmaxf(acc[j] + bias, 0.0f));
         }
-        const float bias = b2[col], wo = __half2float(w3[col]);
-        for (int j = 0; j < 16; ++j) part[j] = fmaf(fmaxf(acc[j] + bias, 0.0f), wo, part[j]);
-    }
-    // sum over the 32 columns held by the lanes of each half-wave
-    for (int j = 0; j < 16; ++j) {
-        float v = part[j];
-        for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-        part[j] = v;
-    }
-    if (lr == 0)
-        for (int j = 0; j < 16; ++j) red[wave][8 * (j / 4) + lk + (j % 4)] = part[j];
-    __syncthreads();
-    if (tid < kMfmaRows && r0 + tid < R) {
-        float v = b3[0];
-        for (int w = 0; w < NT / 64; ++w) v += red[w][tid];
-        y[r0 + tid] = v;
+        __syncthreads();
+        // layer 2 and the output dot product
+        float part[16];
+        for (int j = 0; j < 16; ++j) part[j] = 0.0f;
+        for (int nt = 0; nt < 2; ++nt) {
+            const int col = 64 * wave + 32 * nt + lr;
+            float16_t acc;
+            for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
+            for (int k0 = 0; k0 < H; k0 += 8) {
+                half4_t a, b;
+                for (int k = 0; k < 4; ++k) {
+                    a[k] = h1[lr][k0 + lk + k];
+                    b[k] = hw(W2r, static_cast<long long>(col) * H + k0 + lk + k);
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
+            }
+            const float bias = b2[col], wo = __half2float(w3[col]);
+            for (int j = 0; j < 16; ++j) part[j] = fmaf(fmaxf(acc[j] + bias, 0.0f), wo, part[j]);
+        }
+        // sum over the 32 columns held by the lanes of each half-wave
+        for (int j = 0; j < 16; ++j) {
+            float v = part[j];
+            for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+            part[j] = v;
+        }
+        if (lr == 0)
+            for (int j = 0; j < 16; ++j) red[wave][8 * (j / 4) + lk + (j % 4)] = part[j];
+        __syncthreads();
+        if (tid < kMfmaRows && r0 + tid < R) {
+            float v = b3[0];
+            for (int w = 0; w < NT / 64; ++w) v += red[w][tid];
+            y[r0 + tid] = v;
+        }
+        __syncthreads();
     }
 }
 
@@ -372,7 +376,7 @@ empi_status forward(empi_mlp* m, const float* X, int R, float* y, bool keep) {
 // the matrix-core inference pass (FP16 inputs and hidden layer)
 empi_status forward16(empi_mlp* m, const float* X, int R, const int* Rdev, float* y, hipStream_t s) {
     if (m->din > kMaxDin) return EMPI_UNSUPPORTED;
-    const int blocks = (R + kMfmaRows - 1) / kMfmaRows;
+    const int blocks = std::min(kMfmaMaxBlocks, (R + kMfmaRows - 1) / kMfmaRows);
     hipLaunchKernelGGL(k_mlp_forward_mfma, dim3(blocks), dim3(NT), 0, s, X, R, Rdev, m->din, m->wh.p + m->oW1(),
                        m->w.p + m->ob1(), m->wh.p + m->oW2(), m->w.p + m->ob2(), m->wh.p + m->ow3(),
                        m->w.p + m->ob3(), y);
