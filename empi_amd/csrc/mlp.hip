@@ -136,6 +136,15 @@ __global__ __launch_bounds__(NT) void k_mlp_forward_mfma(const float* __restrict
             __builtin_memcpy(&v, &w[i].data, sizeof v);
             return v;
         };
+        // four consecutive weights in one 8-byte load (i % 4 == 0 in rows of a
+        // multiple of 4 weights)
+        auto hw4 = [](const __half_raw* w, long long i) {
+            const uint2 raw = *reinterpret_cast<const uint2*>(w + i);
+            half4_t v;
+            __builtin_memcpy(&v, &raw, sizeof v);
+            return v;
+        };
+        const bool din4 = din % 4 == 0;
         // layer 1
         for (int nt = 0; nt < 2; ++nt) {
             const int col = 64 * wave + 32 * nt + lr;
@@ -143,10 +152,19 @@ __global__ __launch_bounds__(NT) void k_mlp_forward_mfma(const float* __restrict
             for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
             for (int k0 = 0; k0 < kp; k0 += 8) {
                 half4_t a, b;
-                for (int k = 0; k < 4; ++k) {
-                    a[k] = xs[lr][k0 + lk + k];
-                    const int kk = k0 + lk + k;
-                    b[k] = kk < din ? hw(W1r, static_cast<long long>(col) * din + kk) : static_cast<_Float16>(0.0f);
+                for (int k = 0; k < 4; ++k) a[k] = xs[lr][k0 + lk + k];
+                if (din4) {
+                    if (k0 + lk < din) {
+                        b = hw4(W1r, static_cast<long long>(col) * din + k0 + lk);
+                    } else {
+                        for (int k = 0; k < 4; ++k) b[k] = static_cast<_Float16>(0.0f);
+                    }
+                } else {
+                    for (int k = 0; k < 4; ++k) {
+                        const int kk = k0 + lk + k;
+                        b[k] = kk < din ? hw(W1r, static_cast<long long>(col) * din + kk)
+                                        : static_cast<_Float16>(0.0f);
+                    }
                 }
                 acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
             }
@@ -163,10 +181,8 @@ __global__ __launch_bounds__(NT) void k_mlp_forward_mfma(const float* __restrict
             for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
             for (int k0 = 0; k0 < H; k0 += 8) {
                 half4_t a, b;
-                for (int k = 0; k < 4; ++k) {
-                    a[k] = h1[lr][k0 + lk + k];
-                    b[k] = hw(W2r, static_cast<long long>(col) * H + k0 + lk + k);
-                }
+                for (int k = 0; k < 4; ++k) a[k] = h1[lr][k0 + lk + k];
+                b = hw4(W2r, static_cast<long long>(col) * H + k0 + lk);
                 acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, b, acc, 0, 0, 0);
             }
             const float bias = b2[col], wo = __half2float(w3[col]);
