@@ -19,10 +19,10 @@ constexpr double kMinScaleInSamples = 2.0;     // src/gabor.cpp:23
 // Sub-transforms of the column kernels (complex points): up to 4096 for
 // single-channel units, 8192 for multichannel ones (their channel loop keeps
 // one workgroup per CU either way).  Longer windows fold onto 4096 points, at
-// most 8 sections: windows up to 2 * 4096 * 8 = 65,536 samples.
+// most 32 sections: windows up to 2 * 4096 * 32 = 262,144 samples.
 constexpr int kMaxSubTransform = 4096;
 constexpr int kMaxSubTransformMulti = 8192;
-constexpr int kMaxFold = 8;
+constexpr int kMaxFold = 32;
 
 struct Scale {
     double s;        // scale [s]

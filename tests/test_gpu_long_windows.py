@@ -40,3 +40,16 @@ def test_folded_windows_multichannel(variant):
     assert res.status == 0
     ora = _oracle(x, 256.0, 0.01, variant, 3, 99.0)
     _check_item(res, ora, variant, f"2 x 20000 samples, variant {variant}")
+
+
+def test_nine_folds():
+    """70,000 samples: the largest windows span 35,000 packed points, nine
+    4096-point sections."""
+    _announce("long windows: 70000 samples @256 Hz, 2 iterations (nine folds)")
+    x = synth.pink_with_atoms(4350, 70000, 256.0, 4)
+    spec = E.ItemSpec(1, 70000, 256.0, E.SMP, 2, 99.0, 0.01)
+    plan = _run_plan([spec], [x])
+    res = plan.result(0)
+    assert res.status == 0
+    ora = _oracle(x[None, :], 256.0, 0.01, po.SMP, 2, 99.0)
+    _check_item(res, ora, E.SMP, "70000 samples")
