@@ -663,6 +663,12 @@ __global__ __launch_bounds__(kRegionThreads) void k_block_spectra(const UnitDev*
         b0 = max(0, static_cast<int>(lrint((st.tA - hw) * ud.fs)) / kBlock);
         b1 = min(ud.B - 1, static_cast<int>(lrint((st.tA + hw) * ud.fs)) / kBlock);
     }
+    // e^{-2 pi i j / 256}, the same hardware values per (k n) mod 256 as a
+    // per-product sine/cosine, looked up
+    __shared__ float2 wtab[kBlock];
+    for (int j = threadIdx.x; j < kBlock; j += kRegionThreads)
+        wtab[j] = cis_rev(-static_cast<float>(j) * (1.0f / kBlock));
+    __syncthreads();
     const int waves = kRegionThreads / 64;
     const int m = threadIdx.x & 63;
     for (int b = b0 + blockIdx.x * waves + (threadIdx.x >> 6); b <= b1; b += gridDim.x * waves) {
@@ -673,7 +679,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_block_spectra(const UnitDev*
             for (int n = 0; n < kBlock; ++n) {
                 const int i = b * kBlock + n;
                 const float v = i < N ? x[i] : 0.0f;
-                const float2 w = cis_rev(-static_cast<float>((k * n) & (kBlock - 1)) * (1.0f / kBlock));
+                const float2 w = wtab[(k * n) & (kBlock - 1)];
                 re = fmaf(v, w.x, re);
                 im = fmaf(v, w.y, im);
             }
