@@ -2,8 +2,9 @@
 seed, one plan holding twelve items of random variant, channel count, length,
 sampling rate, energyError, iteration limit and dictionary options, each
 compared with its own oracle run (indices up to the first near-tie,
-amplitudes, phases and energies).  The draws are fixed by the seed, so the
-cases are the same on every run."""
+amplitudes, phases and energies); and the learned proposal at delta = 0 on
+random single-channel items, which must pass the same parity.  The draws are
+fixed by the seed, so the cases are the same on every run."""
 import numpy as np
 import pytest
 
@@ -53,3 +54,32 @@ def test_random_configurations_one_plan(seed):
         assert res.status == 0, where
         ora = _oracle(x, fs, eps, v, it, pc, **kw)
         _check_item(res, ora, v, where)
+
+
+@pytest.mark.parametrize("seed", [7400, 7500])
+def test_random_configurations_proposal_exact(seed):
+    """The learned proposal at delta = 0 (every iteration certified exact) on
+    random single-channel items: the atoms must pass the same oracle parity
+    as the plain method."""
+    from empi_amd import proposal
+    rng = np.random.default_rng(seed)
+    _announce(f"proposal delta=0 on 10 random single-channel items (seed {seed})")
+    cases = []
+    for _ in range(10):
+        N = int(rng.integers(600, 4000))
+        fs = float(rng.choice([128.0, 250.0, 256.0]))
+        eps = float(rng.choice([0.01, 0.02]))
+        iters = int(rng.integers(5, 20))
+        x = synth.pink_with_atoms(int(rng.integers(1 << 30)), N, fs, int(rng.integers(1, 7)))
+        cases.append((N, fs, eps, iters, x))
+    specs = [E.ItemSpec(1, N, fs, E.SMP, it, 99.0, eps) for (N, fs, eps, it, x) in cases]
+    plan = E.Plan(specs)
+    plan.set_proposal(E.PROPOSAL_ON, proposal.load(), top_k=8, delta=0.0)
+    plan.run(np.concatenate([c[-1] for c in cases]).astype(np.float32))
+    for i, (N, fs, eps, it, x) in enumerate(cases):
+        where = f"item {i}: {N} @{fs:g} Hz, eps {eps}, {it} it"
+        res = plan.result(i)
+        assert res.status == 0, where
+        ora = _oracle(x[None, :], fs, eps, E.SMP, it, 99.0)
+        _check_item(res, ora, E.SMP, where)
+    print("stats", plan.proposal_stats(), flush=True)
