@@ -203,9 +203,11 @@ empi_status empi_mlp_train(empi_mlp* mlp, const float* X, const float* target, i
  * Each iteration then evaluates exactly: every dirty region of a scale whose
  * windows span a quarter of the signal, the top_k other dirty regions by
  * network score (by their last exact maximum when net is NULL), and, as the
- * full-search fallback, every dirty region whose certified bound
- * csnorm * (residual energy under its windows) exceeds P / (1 - delta), P the
- * best evaluated product.  The chosen atom is therefore within (1 - delta) of
+ * full-search fallback, every dirty region whose certified bound exceeds
+ * P / (1 - delta), P the best evaluated product.  The bound is the smaller of
+ * csnorm * (residual energy under the region's windows) and the atom-overlap
+ * bound (sqrt(last exact maximum) + sum over the atoms since of
+ * |amplitude| * overlap)^2.  The chosen atom is therefore within (1 - delta) of
  * the exact maximum.  Single-channel units only (SMP, MMP2 sums, one-channel
  * items); EMPI_UNSUPPORTED otherwise.
  *   mode EMPI_PROPOSAL_OFF      the plain method (default)
