@@ -785,11 +785,10 @@ __global__ void k_scores_scatter(const float* __restrict__ y, const int* __restr
         regions[rowmap[i]].rscore = y[i];
 }
 
-// evaluate: every dirty region of a global scale, and the top-K dirty
-// regions of the others by network score (by stale max without a network)
+// evaluate: the top-K dirty candidate regions by network score (by stale
+// maximum without a network); every dirty region in COLLECT mode (all)
 __global__ __launch_bounds__(kRegionThreads) void k_choose(const UnitDev* __restrict__ units,
                                                            const UnitState* __restrict__ state,
-                                                           const ScaleDev* __restrict__ scales,
                                                            RegionState* __restrict__ regions, int K, int by_score,
                                                            int all) {
     __shared__ float bv[kRegionThreads];
@@ -1401,11 +1400,7 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
             s.gA = static_cast<float>(M_PI / ((sc.s * d.fs) * (sc.s * d.fs)));
             s.Wmax = sc.Wmax;
             s.fold = sc.fold;
-            {
-                const double sfs = sc.s * d.fs;
-                s.csnorm = static_cast<float>(proposal_bound_factor(sc.s, d.fs, sc.Nfft));
-                s.global = 6.0 * sfs >= 0.25 * d.N ? 1 : 0;
-            }
+            s.csnorm = static_cast<float>(proposal_bound_factor(sc.s, d.fs, sc.Nfft));
             p->scales_h.push_back(s);
         }
     }
@@ -1667,8 +1662,8 @@ empi_status launch_proposal_iteration(empi_plan* p, hipStream_t s) {
         hipLaunchKernelGGL(k_scores_scatter, dim3(256), dim3(256), 0, s, p->d_scores.p, p->d_rowmap.p,
                            p->d_rowcount.p, p->max_rows, p->d_regions.p);
     }
-    hipLaunchKernelGGL(k_choose, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_scales.p,
-                       p->d_regions.p, p->top_k, scored ? 1 : 0, collect ? 1 : 0);
+    hipLaunchKernelGGL(k_choose, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_regions.p,
+                       p->top_k, scored ? 1 : 0, collect ? 1 : 0);
     EMPI_HIP(hipGetLastError());
     if ((st = proposal_pass(p, s, p->d_pstats.p + 4)) != EMPI_OK) return st;
     if (collect && p->d_traincount.p) {

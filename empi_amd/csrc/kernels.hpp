@@ -33,7 +33,6 @@ struct ScaleDev {
     float invKp;              // 1/K'
     int JT;                   // bins j < JT carry E = exp(-2 pi (j alpha)^2) > 0 in FP32
     const float2* etab;       // [JT]: ((1 - E)/K', 2E/K') in FP32 from FP64
-    int global;               // windows span a quarter of the signal or more
     float csnorm;             // max discrete/continuous atom-norm ratio: P <= csnorm * window energy
 };
 

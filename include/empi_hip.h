@@ -200,8 +200,7 @@ empi_status empi_mlp_train(empi_mlp* mlp, const float* X, const float* target, i
 /* Proposal mode of a plan (no counterpart in the reference).  A region is one
  * scale x one 256-sample block of one signal.  Subtracting an atom marks the
  * regions it overlaps (the reference's |t - tA| <= 3 (s + sA) rule) dirty.
- * Each iteration then evaluates exactly: every dirty region of a scale whose
- * windows span a quarter of the signal, the top_k other dirty regions by
+ * Each iteration then evaluates exactly: the top_k dirty candidate regions by
  * network score (by their last exact maximum when net is NULL), and, as the
  * full-search fallback, every dirty region whose certified bound exceeds
  * P / (1 - delta), P the best evaluated product.  The bound is the smaller of
