@@ -485,7 +485,8 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
 // Regions: (scale, 256-sample block) of one unit.  A region is clean while
 // every cell in it holds its exact product; subtracting an atom makes the
 // regions it overlaps (the reference's own rule) dirty.  Per iteration: the
-// network scores the dirty regions and the top-K are evaluated exactly; then
+// network scores the dirty regions whose bound exceeds the best clean product
+// (the candidates) and the top-K are evaluated exactly; then
 // every dirty region whose bound exceeds P* / (1 - delta), P* the best clean
 // product, is evaluated too (the exact fallback).  A region's bound is the
 // smaller of a Cauchy-Schwarz bound, csnorm * (residual energy under its
