@@ -17,8 +17,9 @@ constexpr int kMaxLgQ = 13;   // Q = 8192
 constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
 
 // Workgroup geometry of class Q: NSL slots of Q complex points in LDS (4096
-// points, two slots from Q = 2048 on, at most 512 slots), 16 points per
-// thread.  Shared by the kernels and the host's grid and tile arithmetic.
+// points in at most 512 slots up to Q = 1024, two slots from Q = 2048 on),
+// 16 points per thread.  Shared by the kernels and the host's grid and tile
+// arithmetic.
 constexpr int kClassPoints = 4096;
 constexpr int class_slots(int Q) {
     return Q >= 2048 ? 2 : (kClassPoints / Q > 512 ? 512 : kClassPoints / Q);
