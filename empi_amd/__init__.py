@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 

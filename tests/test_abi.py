@@ -42,7 +42,6 @@ def test_strerror_codes_match_reference_strings():
 
 
 def test_host_side_argument_checks():
-    import math
     with pytest.raises(E.EmpiError) as ei:
         E.dictionary(128.0, 1280, 1.5)
     assert ei.value.status == 4

@@ -3,7 +3,6 @@ stdout protocol out, checked against the FP64 oracle and the reference's
 output conventions (src/empi.cpp:132-164, src/classes.cpp:41-48,83)."""
 import os
 import subprocess
-import sys
 
 import numpy as np
 import pytest

@@ -3,7 +3,6 @@
 Each case prints its name before touching the GPU.  Sizes are chosen so the
 oracle finishes in seconds.
 """
-import math
 import sys
 
 import numpy as np

@@ -8,7 +8,6 @@
 * delta = 1e-3: explained-energy deficit against the plain run <= 0.1%,
   with fewer map entries evaluated.
 """
-import sys
 
 import numpy as np
 import pytest

@@ -1,7 +1,6 @@
 """Dictionary construction (src/gabor.cpp:213-252) of the oracle and of the
 HIP library's host code, against the closed-form table (golden fixture)."""
 import json
-import math
 import os
 
 import pytest
