@@ -676,10 +676,15 @@ static int ws_find_best(workspace* w, eo_atom* out, double* gap) {
     return EO_OK;
 }
 
+/* planted mistake for the tests (eo_plant_narrow_dirty_rule) */
+static int g_narrow_dirty_rule = 0;
+
+void eo_plant_narrow_dirty_rule(int on) { g_narrow_dirty_rule = on != 0; }
+
 /* GaborWorkspace::subtractAtom (gabor.cpp:189-209) */
 static void ws_subtract(workspace* w, eo_atom* atom, double* residual, int c) {
     const eo_dict* d = w->d;
-    const double sA = atom->params[3] / d->fs;
+    const double sA = g_narrow_dirty_rule ? 0.0 : atom->params[3] / d->fs;
     const double tA = atom->params[2] / d->fs;
     eo_subtract(d->fs, atom, residual, d->N, 0);
     const int L = d->L;
@@ -691,6 +696,46 @@ static void ws_subtract(workspace* w, eo_atom* atom, double* residual, int c) {
                 map_column(w, m, residual, c, t);
         }
     }
+}
+
+/* the stored maps of every scale, for a trace callback */
+static void trace_call(workspace* w, eo_trace_fn fn, void* user, int iteration, int row, const double* residual) {
+    const int L = w->d->L;
+    eo_map_view* views = (eo_map_view*)calloc(L > 0 ? (size_t)L : 1, sizeof(eo_map_view));
+    if (!views) return;
+    for (int k = 0; k < L; ++k) {
+        views[k].map = w->maps[k].map;
+        views[k].C = w->maps[k].C;
+        views[k].F = w->maps[k].F;
+        views[k].T = w->maps[k].T;
+    }
+    fn(user, iteration, row, residual, w->C, w->d->N, views, L);
+    free(views);
+}
+
+void eo_fresh_products(const eo_dict* d, int k, const double* x, int C, int constraint, double* out) {
+    const eo_scale* sc = &d->scales[k];
+    eo_fft_plan p;
+    double* buf = (double*)malloc(sizeof(double) * (size_t)sc->Nfft);
+    double* X = (double*)malloc(sizeof(double) * 2 * (size_t)(sc->Nfft / 2 + 1));
+    double* z = (double*)malloc(sizeof(double) * 2 * (size_t)(sc->Nfft / 2 + 1));
+    double* cols = (double*)malloc(sizeof(double) * 2 * (size_t)sc->F * (size_t)C);
+    double* v = (double*)malloc(sizeof(double) * 2 * (size_t)C);
+    if (plan_init(&p, sc->Nfft) == EO_OK && buf && X && z && cols && v) {
+        for (int t = 0; t < sc->T; ++t) {
+            for (int c = 0; c < C; ++c)
+                column_plan(d, k, &p, x + (size_t)c * d->N, t, cols + 2 * (size_t)c * sc->F, buf, X, z);
+            for (int f = 0; f < sc->F; ++f) {
+                for (int c = 0; c < C; ++c) {
+                    v[2 * c] = cols[2 * ((size_t)c * sc->F + f)];
+                    v[2 * c + 1] = cols[2 * ((size_t)c * sc->F + f) + 1];
+                }
+                out[(size_t)f * sc->T + t] = eo_product(d, k, f, t, v, C, constraint);
+            }
+        }
+    }
+    plan_free(&p);
+    free(buf); free(X); free(z); free(cols); free(v);
 }
 
 /* SingleSignal/MultiSignal::computeEnergy (base.hpp:39-58) */
@@ -707,7 +752,8 @@ static double energy_of(const double* x, int C, int N, double fs) {
 /* Decomposition::compute (classes.cpp:26-73) on a C-channel signal.  atoms
  * [C][I] with row stride I; energies / gaps one row. */
 static int mp_loop(workspace* w, const double* signal, int C, const eo_settings* st, int iterLimit,
-                   eo_atom* atoms, int I, int* count, double* energies, double* gaps) {
+                   eo_atom* atoms, int I, int* count, double* energies, double* gaps,
+                   eo_trace_fn trace, void* user, int row) {
     const eo_dict* d = w->d;
     const int N = d->N;
     int status = ws_compute(w, signal);
@@ -731,6 +777,7 @@ static int mp_loop(workspace* w, const double* signal, int C, const eo_settings*
         *count = iteration;
         if (iteration == st->iterationMax) break;
         for (int c = 0; c < C; ++c) ws_subtract(w, &best[c], residue + (size_t)c * N, c);
+        if (trace) trace_call(w, trace, user, iteration, row, residue);
         residueEnergy = energy_of(residue, C, N, d->fs);
         energies[iteration] = residueEnergy;
         if (residueEnergy / totalEnergy <= st->residualEnergy) break;
@@ -743,6 +790,12 @@ static int mp_loop(workspace* w, const double* signal, int C, const eo_settings*
 int eo_decompose_limited(const eo_dict* d, const double* signal, int C, const eo_settings* st,
                          eo_atom* atoms, int* counts, double* energies, double* gaps,
                          int threads, int iterLimit) {
+    return eo_decompose_traced(d, signal, C, st, atoms, counts, energies, gaps, threads, iterLimit, NULL, NULL);
+}
+
+int eo_decompose_traced(const eo_dict* d, const double* signal, int C, const eo_settings* st,
+                        eo_atom* atoms, int* counts, double* energies, double* gaps,
+                        int threads, int iterLimit, eo_trace_fn trace, void* user) {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
@@ -763,7 +816,7 @@ int eo_decompose_limited(const eo_dict* d, const double* signal, int C, const eo
         if (status != EO_OK) return status;
         for (int c = 0; c < C && status == EO_OK; ++c) {
             status = mp_loop(&w, signal + (size_t)c * N, 1, st, iterLimit, atoms + (size_t)c * I, I,
-                             &counts[c], energies + (size_t)c * (I + 1), gaps + (size_t)c * I);
+                             &counts[c], energies + (size_t)c * (I + 1), gaps + (size_t)c * I, trace, user, c);
         }
         ws_free(&w);
     } else if (st->variant == EO_MMP2) {
@@ -777,7 +830,7 @@ int eo_decompose_limited(const eo_dict* d, const double* signal, int C, const eo
         status = ws_init(&w, d, 1, 0);
         if (status == EO_OK) {
             int n = 0;
-            status = mp_loop(&w, sum, 1, st, iterLimit, atoms, I, &n, energies, gaps);
+            status = mp_loop(&w, sum, 1, st, iterLimit, atoms, I, &n, energies, gaps, trace, user, 0);
             ws_free(&w);
             if (status == EO_OK) {
                 memcpy(residue, signal, sizeof(double) * (size_t)C * N);
@@ -804,7 +857,7 @@ int eo_decompose_limited(const eo_dict* d, const double* signal, int C, const eo
         status = ws_init(&w, d, C, st->variant == EO_MMP1);
         if (status != EO_OK) return status;
         int n = 0;
-        status = mp_loop(&w, signal, C, st, iterLimit, atoms, I, &n, energies, gaps);
+        status = mp_loop(&w, signal, C, st, iterLimit, atoms, I, &n, energies, gaps, trace, user, 0);
         for (int c = 0; c < C; ++c) counts[c] = n;
         ws_free(&w);
     } else {

@@ -23,8 +23,10 @@
  * columns against a brute-force windowed DFT; products against the closed
  * form of analytic Gabor atoms; the heap against the reference heap.hpp
  * compiled from its own source (oracle/_ref) on tie-heavy sequences; the
- * MMP1 constraint against its invariants; the loop against brute-force
- * argmax, energy monotonicity and least-squares invariants.
+ * MMP1 constraint against its invariants; the loop at every iteration
+ * against a brute-force argmax over fresh products of the residual and its
+ * stored maps against a full rebuild (a planted narrow re-evaluation rule
+ * fails both), energy monotonicity and least-squares invariants.
  */
 #ifndef EMPI_ORACLE_H
 #define EMPI_ORACLE_H
@@ -130,6 +132,37 @@ int eo_decompose_limited(const eo_dict* d, const double* signal, int C, const eo
                          int threads, int iterLimit);
 
 const char* eo_strerror(int code);
+
+/* ------------------------------------------------------------ loop pinning
+ * Hooks for the tests that pin the lazy re-evaluation of the loop
+ * (gabor.cpp:189-209, 314-354) against a from-scratch computation. */
+
+/* The stored map of one scale: [C][F][T] complex, interleaved. */
+typedef struct {
+    const double* map;
+    int C, F, T;
+    int pad;
+} eo_map_view;
+
+/* Called once per MP iteration that subtracts, after the subtraction of every
+ * channel and before the lazy index update of the next search: iteration =
+ * atoms subtracted so far (1-based), row = the SMP channel (else 0), the
+ * residual [C][N] and the stored maps of all L scales. */
+typedef void (*eo_trace_fn)(void* user, int iteration, int row, const double* residual, int C, int N,
+                            const eo_map_view* maps, int L);
+
+/* eo_decompose_limited with a trace callback (fn may be NULL). */
+int eo_decompose_traced(const eo_dict* d, const double* signal, int C, const eo_settings* st,
+                        eo_atom* atoms, int* counts, double* energies, double* gaps,
+                        int threads, int iterLimit, eo_trace_fn fn, void* user);
+
+/* Products of every entry of scale k computed from fresh columns of x [C][N]
+ * (column + product only: no stored map, heap or lazy index), out[f*T + t]. */
+void eo_fresh_products(const eo_dict* d, int k, const double* x, int C, int constraint, double* out);
+
+/* Planted mistake for the tests: on != 0 narrows subtractAtom's
+ * re-evaluation rule from |t - tA| <= 3 (s + sA) to |t - tA| <= 3 s. */
+void eo_plant_narrow_dirty_rule(int on);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,15 @@ class _Settings(C.Structure):
                 ("variant", C.c_int)]
 
 
+class _MapView(C.Structure):
+    _fields_ = [("map", C.POINTER(C.c_double)), ("C", C.c_int), ("F", C.c_int), ("T", C.c_int),
+                ("pad", C.c_int)]
+
+
+_TRACE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int,
+                        C.POINTER(_MapView), C.c_int)
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -69,6 +78,11 @@ def lib():
         L.eo_decompose_limited.argtypes = [C.POINTER(_Dict), dp, C.c_int, C.POINTER(_Settings),
                                            C.POINTER(_Atom), C.POINTER(C.c_int), dp, dp, C.c_int,
                                            C.c_int]
+        L.eo_decompose_traced.argtypes = [C.POINTER(_Dict), dp, C.c_int, C.POINTER(_Settings),
+                                          C.POINTER(_Atom), C.POINTER(C.c_int), dp, dp, C.c_int,
+                                          C.c_int, _TRACE_FN, C.c_void_p]
+        L.eo_fresh_products.argtypes = [C.POINTER(_Dict), C.c_int, dp, C.c_int, C.c_int, dp]
+        L.eo_plant_narrow_dirty_rule.argtypes = [C.c_int]
         L.eo_strerror.restype = C.c_char_p
         _LIB = L
     return _LIB
@@ -122,6 +136,15 @@ class Dictionary:
         v[0::2] = np.real(values)
         v[1::2] = np.imag(values)
         return lib().eo_product(C.byref(self._d), k, f, t, _dp(v), len(values), int(constraint))
+
+    def fresh_products(self, k, x, constraint=False):
+        """Products [F][T] of scale k from fresh columns of x [C][N]: column +
+        product only, no stored map or lazy index."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        sc = self.scales[k]
+        out = np.zeros(sc.F * sc.T)
+        lib().eo_fresh_products(C.byref(self._d), k, _dp(x), x.shape[0], int(constraint), _dp(out))
+        return out.reshape(sc.F, sc.T)
 
     def atom_params(self, k, f, t, values):
         v = np.empty(2 * len(values))
@@ -200,8 +223,22 @@ class Result:
     status: int
 
 
+@dataclass
+class TraceStep:
+    iteration: int          # atoms subtracted so far
+    row: int                # SMP channel, else 0
+    residual: np.ndarray    # [C][N] after the subtractions
+    maps: list              # per scale: stored map [C][F][T] complex, before the lazy update
+
+
+def plant_narrow_dirty_rule(on):
+    """Planted mistake (tests only): re-evaluate |t - tA| <= 3 s instead of 3 (s + sA)."""
+    lib().eo_plant_narrow_dirty_rule(int(on))
+
+
 def decompose(d: Dictionary, signal, variant, iteration_max, energy_percent,
-              threads=0, iter_limit=None, check=True) -> Result:
+              threads=0, iter_limit=None, check=True, trace=None) -> Result:
+    """trace: a list that receives one TraceStep per subtracting iteration."""
     sig = np.ascontiguousarray(np.atleast_2d(signal), dtype=np.float64)
     Cn, N = sig.shape
     assert N == d.N
@@ -212,10 +249,20 @@ def decompose(d: Dictionary, signal, variant, iteration_max, energy_percent,
     rows = Cn if variant == SMP else 1
     energies = np.zeros(rows * (I + 1))
     gaps = np.zeros(rows * I)
-    status = lib().eo_decompose_limited(
+    def on_step(_user, iteration, row, residual, nc, n, views, L):
+        res = np.ctypeslib.as_array(residual, shape=(nc * n,)).reshape(nc, n).copy()
+        maps = []
+        for k in range(L):
+            v = views[k]
+            a = np.ctypeslib.as_array(v.map, shape=(2 * v.C * v.F * v.T,))
+            maps.append((a[0::2] + 1j * a[1::2]).reshape(v.C, v.F, v.T).copy())
+        trace.append(TraceStep(iteration, row, res, maps))
+
+    cb = _TRACE_FN(on_step) if trace is not None else _TRACE_FN()
+    status = lib().eo_decompose_traced(
         C.byref(d._d), _dp(sig), Cn, C.byref(st), atoms,
         counts.ctypes.data_as(C.POINTER(C.c_int)), _dp(energies), _dp(gaps), int(threads),
-        int(iter_limit if iter_limit is not None else 2**31 - 1))
+        int(iter_limit if iter_limit is not None else 2**31 - 1), cb, None)
     if check and status != 0:
         raise OracleError(status)
     params = np.zeros((Cn, I, 6))

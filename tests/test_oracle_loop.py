@@ -1,5 +1,8 @@
-"""MP loop of the oracle (src/classes.cpp:26-87, src/mmp.cpp:23-45):
-brute-force argmax, stop rule, energy bookkeeping and variant semantics."""
+"""MP loop of the oracle (src/classes.cpp:26-87, src/mmp.cpp:23-45,
+src/gabor.cpp:189-209, 314-354): at every iteration the stored maps against a
+full rebuild and the chosen atom against a brute-force argmax (with a
+planted too-narrow re-evaluation rule that both must catch), the stop rule,
+energy bookkeeping and variant semantics."""
 import numpy as np
 import pytest
 
@@ -109,3 +112,158 @@ def test_mmp1_shares_phase_across_channels():
         ph = r.params[:, i, 5]
         diff = np.angle(np.exp(2j * (ph - ph[0])))   # equal modulo pi
         assert np.max(np.abs(diff)) < 1e-9
+
+
+# ---------------------------------------------------------------- every iteration
+#
+# The reference re-evaluates, after each subtraction, the columns with
+# |t - tA| <= 3 (s + sA) of every scale (gabor.cpp:189-209) and updates its
+# heaps lazily before the next search (gabor.cpp:314-325, 346-354).  The two
+# checks below pin that path against a computation from scratch at every
+# iteration: (a) the stored map equals a full rebuild of the new residual,
+# (b) the next atom is the brute-force argmax of fresh products.
+
+def _window(d, k, t):
+    """Samples [iL, iR] of column t of scale k (gabor.cpp:284-286); np.rint is
+    round-half-even like lrint."""
+    sc = d.scales[k]
+    t0 = t * d.tMax / (sc.T - 1)
+    iL = int(np.rint((t0 - 3 * sc.s) * d.fs))
+    iR = int(np.rint((t0 + 3 * sc.s) * d.fs))
+    return max(iL, 0), min(iR, d.N - 1), t0
+
+
+def _support(d, params):
+    """Samples [iL, iR] a subtraction changes (gabor.cpp:152-157)."""
+    sA, tA = params[3] / d.fs, params[2] / d.fs
+    iL = int(np.rint((tA - 3 * sA) * d.fs))
+    iR = int(np.rint((tA + 3 * sA) * d.fs))
+    return max(iL, 0), min(iR, d.N - 1), tA, sA
+
+
+def _edge_contact(d, k, t, atoms):
+    """True if, for some subtracted atom the predicate |t - tA| <= 3 (s + sA)
+    excludes, the column's window shares exactly one sample with the atom's
+    support (the lrint boundary sample)."""
+    wl, wr, t0 = _window(d, k, t)
+    s = d.scales[k].s
+    for p in atoms:
+        al, ar, tA, sA = _support(d, p)
+        if abs(t0 - tA) <= 3 * (s + sA):
+            continue
+        if min(wr, ar) - max(wl, al) == 0:
+            return True
+    return False
+
+
+def check_map_rebuild(d, step, atoms, tol=1e-12):
+    """(a): returns (columns compared, columns not bit-identical, worst
+    relative difference).  A column may differ from its rebuild only through
+    an edge contact, and then by at most tol of its magnitude."""
+    compared, inexact, worst = 0, 0, 0.0
+    for k, m in enumerate(step.maps):
+        C, F, T = m.shape
+        for c in range(C):
+            for t in range(T):
+                fresh = d.column(k, step.residual[c], t)
+                compared += 1
+                diff = np.max(np.abs(m[c, :, t] - fresh))
+                if diff == 0.0:
+                    continue
+                inexact += 1
+                rel = diff / max(np.max(np.abs(fresh)), 1e-300)
+                worst = max(worst, rel)
+                assert rel <= tol and _edge_contact(d, k, t, atoms), (
+                    f"iteration {step.iteration} row {step.row}: scale {k} column {t} channel {c} "
+                    f"differs from its rebuild by {rel:.3e} relative")
+    return compared, inexact, worst
+
+
+def brute_force_top2(d, x, constraint):
+    """Best and runner-up products over every entry of fresh columns of x [C][N]."""
+    best, second, idx = -1.0, -1.0, None
+    for k in range(len(d.scales)):
+        P = d.fresh_products(k, x, constraint)
+        flat = P.ravel()
+        order = np.argsort(flat)[::-1][:2]
+        f, t = divmod(int(order[0]), P.shape[1])
+        v0 = flat[order[0]]
+        v1 = flat[order[1]] if len(order) > 1 else -1.0
+        if v0 > best:
+            second = max(best, v1)
+            best, idx = v0, (k, f, t)
+        else:
+            second = max(second, v0)
+    return best, second, idx
+
+
+LOOP_CASES = [
+    # name, variant, channels, N, fs, eps, iterations, seed
+    ("smp", po.SMP, 2, 256, 64.0, 0.05, 10, 910),
+    ("mmp1", po.MMP1, 2, 256, 64.0, 0.05, 8, 911),
+    ("mmp2", po.MMP2, 2, 256, 64.0, 0.05, 8, 912),
+    ("mmp3", po.MMP3, 2, 256, 64.0, 0.05, 8, 913),
+]
+
+
+def _loop_signal(variant, C, N, fs, seed):
+    if variant == po.SMP:
+        return np.stack([synth.pink_with_atoms(seed + c, N, fs, 3) for c in range(C)]).astype(np.float64)
+    return synth.shared_multichannel(C=C, N=N, fs=fs, atoms=3, noise_sigma=0.2, seed=seed).astype(np.float64)
+
+
+def run_loop_checks(name, variant, C, N, fs, eps, iters, seed, rebuild=True, argmax=True):
+    """Decomposes with a trace and runs (a) (rebuild) and (b) (argmax) at
+    every subtracting iteration; returns the number of iterations checked."""
+    x = _loop_signal(variant, C, N, fs, seed)
+    d = po.Dictionary(fs, N, eps)
+    trace = []
+    r = po.decompose(d, x, variant, iters, 100.0, trace=trace)
+    constraint = variant == po.MMP1
+    checked = 0
+    for step in trace:
+        row, i = step.row, step.iteration
+        if variant == po.SMP:
+            sub = [r.params[row, j] for j in range(i)]
+            nxt_row = row
+        elif variant == po.MMP2:
+            # the trace holds the channel sum's loop; its atoms are the refits' positions
+            sub = [r.params[0, j] for j in range(i)]
+            nxt_row = 0
+        else:
+            sub = [r.params[c, j] for c in range(C) for j in range(i)]
+            nxt_row = 0
+        if rebuild:
+            check_map_rebuild(d, step, sub)
+        if argmax and i < int(r.counts[nxt_row]):
+            best, second, idx = brute_force_top2(d, step.residual, constraint)
+            gap = (best - second) / best
+            if gap > 0:
+                assert tuple(r.index[nxt_row, i]) == idx, (name, row, i, r.index[nxt_row, i], idx, gap)
+            assert abs(r.gaps[row if variant == po.SMP else 0, i] - gap) <= 1e-9, (name, row, i)
+        checked += 1
+    return checked
+
+
+@pytest.mark.parametrize("case", LOOP_CASES, ids=[c[0] for c in LOOP_CASES])
+def test_loop_matches_rebuild_and_brute_force_every_iteration(case):
+    name, variant, C, N, fs, eps, iters, seed = case
+    rows = C if variant == po.SMP else 1
+    checked = run_loop_checks(*case)
+    assert checked == rows * (iters - 1), (name, checked)
+
+
+@pytest.mark.parametrize("case", LOOP_CASES, ids=[c[0] for c in LOOP_CASES])
+def test_planted_narrow_dirty_rule_is_caught(case):
+    """Each check on its own fails when the oracle re-evaluates only
+    |t - tA| <= 3 s: the rebuild finds stale columns, and the brute-force
+    argmax disagrees with the stale map's choice."""
+    po.plant_narrow_dirty_rule(True)
+    try:
+        with pytest.raises(AssertionError) as ei:
+            run_loop_checks(*case, argmax=False)
+        assert "differs from its rebuild" in str(ei.value)
+        with pytest.raises(AssertionError):
+            run_loop_checks(*case, rebuild=False)
+    finally:
+        po.plant_narrow_dirty_rule(False)
