@@ -501,6 +501,25 @@ void eo_subtract(double fs, eo_atom* atom, double* x, int Nint, int fit) {
     free(w);
 }
 
+void eo_waveform_norm2(double fs, int N, const double* params, int n, double* out) {
+    double* x = (double*)calloc(N > 0 ? (size_t)N : 1, sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        out[i] = NAN;
+        if (!x) continue;
+        eo_atom a;
+        memset(&a, 0, sizeof a);
+        for (int j = 0; j < 6; ++j) a.params[j] = params[6 * (size_t)i + j];
+        eo_subtract(fs, &a, x, N, 0);
+        double sum = 0.0;
+        for (int m = 0; m < N; ++m) {
+            sum += x[m] * x[m];
+            x[m] = 0.0;
+        }
+        out[i] = sum;
+    }
+    free(x);
+}
+
 /* ---------------------------------------------------------------- workspace */
 
 typedef struct {
@@ -637,8 +656,10 @@ static int ws_compute(workspace* w, const double* signal /*[C][N]*/) {
 }
 
 /* GaborWorkspace::findBestMatch (gabor.cpp:85-133); also reports the
- * runner-up product for the near-tie measure. */
-static int ws_find_best(workspace* w, eo_atom* out, double* gap) {
+ * runner-up product for the near-tie measure.  follow (k, f, t) or NULL: an
+ * entry to take instead of the best one when its product is at least
+ * (1 - tol) of the best (*followed is set then). */
+static int ws_find_best(workspace* w, eo_atom* out, double* gap, const int* follow, double tol, int* followed) {
     const int L = w->d->L;
     double* top = (double*)malloc(sizeof(double) * (L > 0 ? (size_t)L : 1));
     double* second = (double*)malloc(sizeof(double) * (L > 0 ? (size_t)L : 1));
@@ -664,6 +685,22 @@ static int ws_find_best(workspace* w, eo_atom* out, double* gap) {
     ws_map* m = &w->maps[bestK];
     int f = (int)(keys[bestK] / (size_t)m->T);
     int t = (int)(keys[bestK] % (size_t)m->T);
+    if (followed) *followed = 0;
+    if (follow) {
+        const int fk = follow[0], ff = follow[1], ft = follow[2];
+        if (fk >= 0 && fk < L && ff >= 0 && ff < w->maps[fk].F && ft >= 0 && ft < w->maps[fk].T &&
+            (fk != bestK || ff != f || ft != t)) {
+            computer gf = computer_of(w->d, fk, ff);
+            const double pf = map_product(w, &w->maps[fk], &gf, ff, ft);
+            if (pf >= bestV * (1.0 - tol)) {
+                bestK = fk;
+                m = &w->maps[fk];
+                f = ff;
+                t = ft;
+                if (followed) *followed = 1;
+            }
+        }
+    }
     computer g = computer_of(w->d, bestK, f);
     for (int c = 0; c < w->C; ++c) {
         double* src = map_at(m, c, f, t);
@@ -751,9 +788,17 @@ static double energy_of(const double* x, int C, int N, double fs) {
 
 /* Decomposition::compute (classes.cpp:26-73) on a C-channel signal.  atoms
  * [C][I] with row stride I; energies / gaps one row. */
+typedef struct {
+    eo_trace_fn trace;
+    void* user;
+    int row;
+    const int* follow;     /* [I][3] of this row, or NULL */
+    double tol;
+    int* followed;         /* [I] of this row, or NULL */
+} loop_ctl;
+
 static int mp_loop(workspace* w, const double* signal, int C, const eo_settings* st, int iterLimit,
-                   eo_atom* atoms, int I, int* count, double* energies, double* gaps,
-                   eo_trace_fn trace, void* user, int row) {
+                   eo_atom* atoms, int I, int* count, double* energies, double* gaps, const loop_ctl* ctl) {
     const eo_dict* d = w->d;
     const int N = d->N;
     int status = ws_compute(w, signal);
@@ -770,14 +815,17 @@ static int mp_loop(workspace* w, const double* signal, int C, const eo_settings*
     int limit = st->iterationMax < iterLimit ? st->iterationMax : iterLimit;
     for (int iteration = 1; iteration <= limit; ++iteration) {
         double gap = NAN;
-        status = ws_find_best(w, best, &gap);
+        int fl = 0;
+        status = ws_find_best(w, best, &gap, ctl->follow ? ctl->follow + 3 * (size_t)(iteration - 1) : NULL,
+                              ctl->tol, &fl);
+        if (ctl->followed) ctl->followed[iteration - 1] = fl;
         if (status != EO_OK) break;
         gaps[iteration - 1] = gap;
         for (int c = 0; c < C; ++c) atoms[(size_t)c * I + iteration - 1] = best[c];
         *count = iteration;
         if (iteration == st->iterationMax) break;
         for (int c = 0; c < C; ++c) ws_subtract(w, &best[c], residue + (size_t)c * N, c);
-        if (trace) trace_call(w, trace, user, iteration, row, residue);
+        if (ctl->trace) trace_call(w, ctl->trace, ctl->user, iteration, ctl->row, residue);
         residueEnergy = energy_of(residue, C, N, d->fs);
         energies[iteration] = residueEnergy;
         if (residueEnergy / totalEnergy <= st->residualEnergy) break;
@@ -796,6 +844,14 @@ int eo_decompose_limited(const eo_dict* d, const double* signal, int C, const eo
 int eo_decompose_traced(const eo_dict* d, const double* signal, int C, const eo_settings* st,
                         eo_atom* atoms, int* counts, double* energies, double* gaps,
                         int threads, int iterLimit, eo_trace_fn trace, void* user) {
+    return eo_decompose_follow(d, signal, C, st, atoms, counts, energies, gaps, threads, iterLimit, trace, user,
+                               NULL, 0.0, NULL);
+}
+
+int eo_decompose_follow(const eo_dict* d, const double* signal, int C, const eo_settings* st,
+                        eo_atom* atoms, int* counts, double* energies, double* gaps,
+                        int threads, int iterLimit, eo_trace_fn trace, void* user,
+                        const int* follow, double tol, int* followed) {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
@@ -808,6 +864,9 @@ int eo_decompose_traced(const eo_dict* d, const double* signal, int C, const eo_
     for (size_t i = 0; i < (size_t)rows * (I + 1); ++i) energies[i] = NAN;
     for (size_t i = 0; i < (size_t)rows * I; ++i) gaps[i] = NAN;
     for (int c = 0; c < C; ++c) counts[c] = 0;
+    if (followed)
+        for (size_t i = 0; i < (size_t)rows * I; ++i) followed[i] = 0;
+    loop_ctl ctl = {trace, user, 0, follow, tol, followed};
     workspace w;
     int status = EO_OK;
     if (st->variant == EO_SMP) {
@@ -815,8 +874,12 @@ int eo_decompose_traced(const eo_dict* d, const double* signal, int C, const eo_
         status = ws_init(&w, d, 1, 0);
         if (status != EO_OK) return status;
         for (int c = 0; c < C && status == EO_OK; ++c) {
+            loop_ctl cc = ctl;
+            cc.row = c;
+            cc.follow = follow ? follow + 3 * (size_t)c * I : NULL;
+            cc.followed = followed ? followed + (size_t)c * I : NULL;
             status = mp_loop(&w, signal + (size_t)c * N, 1, st, iterLimit, atoms + (size_t)c * I, I,
-                             &counts[c], energies + (size_t)c * (I + 1), gaps + (size_t)c * I, trace, user, c);
+                             &counts[c], energies + (size_t)c * (I + 1), gaps + (size_t)c * I, &cc);
         }
         ws_free(&w);
     } else if (st->variant == EO_MMP2) {
@@ -830,7 +893,7 @@ int eo_decompose_traced(const eo_dict* d, const double* signal, int C, const eo_
         status = ws_init(&w, d, 1, 0);
         if (status == EO_OK) {
             int n = 0;
-            status = mp_loop(&w, sum, 1, st, iterLimit, atoms, I, &n, energies, gaps, trace, user, 0);
+            status = mp_loop(&w, sum, 1, st, iterLimit, atoms, I, &n, energies, gaps, &ctl);
             ws_free(&w);
             if (status == EO_OK) {
                 memcpy(residue, signal, sizeof(double) * (size_t)C * N);
@@ -857,7 +920,7 @@ int eo_decompose_traced(const eo_dict* d, const double* signal, int C, const eo_
         status = ws_init(&w, d, C, st->variant == EO_MMP1);
         if (status != EO_OK) return status;
         int n = 0;
-        status = mp_loop(&w, signal, C, st, iterLimit, atoms, I, &n, energies, gaps, trace, user, 0);
+        status = mp_loop(&w, signal, C, st, iterLimit, atoms, I, &n, energies, gaps, &ctl);
         for (int c = 0; c < C; ++c) counts[c] = n;
         ws_free(&w);
     } else {

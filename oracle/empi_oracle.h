@@ -102,6 +102,10 @@ void eo_atom_params(const eo_dict* d, int k, int f, int t, const double* v, int 
 /* Residual update; fit=1 rewrites params[0..1] (least-squares refit). */
 void eo_subtract(double fs, eo_atom* atom, double* x, int N, int fit);
 
+/* ||w||^2 of the waveforms eo_subtract removes (fit = 0) for n atoms given by
+ * their parameters params [n][6] on an N-sample signal. */
+void eo_waveform_norm2(double fs, int N, const double* params, int n, double* out);
+
 /* Indexed max-heap with the reference's sift rules. */
 typedef struct eo_heap eo_heap;
 eo_heap* eo_heap_new(const double* values, size_t n);
@@ -155,6 +159,16 @@ typedef void (*eo_trace_fn)(void* user, int iteration, int row, const double* re
 int eo_decompose_traced(const eo_dict* d, const double* signal, int C, const eo_settings* st,
                         eo_atom* atoms, int* counts, double* energies, double* gaps,
                         int threads, int iterLimit, eo_trace_fn fn, void* user);
+
+/* eo_decompose_traced that follows a given path through near-ties: follow
+ * [R][I][3] holds (k, f, t) per row and iteration (R as for energies).
+ * Where that entry differs from the oracle's own best and its product is at
+ * least (1 - tol) of the best, the loop takes it instead (a near-tie branch;
+ * followed [R][I] records 1 there); otherwise the loop keeps its own best. */
+int eo_decompose_follow(const eo_dict* d, const double* signal, int C, const eo_settings* st,
+                        eo_atom* atoms, int* counts, double* energies, double* gaps,
+                        int threads, int iterLimit, eo_trace_fn fn, void* user,
+                        const int* follow, double tol, int* followed);
 
 /* Products of every entry of scale k computed from fresh columns of x [C][N]
  * (column + product only: no stored map, heap or lazy index), out[f*T + t]. */

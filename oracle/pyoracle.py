@@ -81,6 +81,11 @@ def lib():
         L.eo_decompose_traced.argtypes = [C.POINTER(_Dict), dp, C.c_int, C.POINTER(_Settings),
                                           C.POINTER(_Atom), C.POINTER(C.c_int), dp, dp, C.c_int,
                                           C.c_int, _TRACE_FN, C.c_void_p]
+        L.eo_decompose_follow.argtypes = [C.POINTER(_Dict), dp, C.c_int, C.POINTER(_Settings),
+                                          C.POINTER(_Atom), C.POINTER(C.c_int), dp, dp, C.c_int,
+                                          C.c_int, _TRACE_FN, C.c_void_p, C.POINTER(C.c_int), C.c_double,
+                                          C.POINTER(C.c_int)]
+        L.eo_waveform_norm2.argtypes = [C.c_double, C.c_int, dp, C.c_int, dp]
         L.eo_fresh_products.argtypes = [C.POINTER(_Dict), C.c_int, dp, C.c_int, C.c_int, dp]
         L.eo_plant_narrow_dirty_rule.argtypes = [C.c_int]
         L.eo_strerror.restype = C.c_char_p
@@ -187,6 +192,14 @@ def subtract(fs, params, x, fit=False):
     return x, np.array(list(a.params))
 
 
+def waveform_norm2(fs, N, params):
+    """||w||^2 of the waveform subtract() removes, per atom of params [n][6]."""
+    p = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+    out = np.zeros(p.shape[0])
+    lib().eo_waveform_norm2(float(fs), int(N), _dp(p), p.shape[0], _dp(out))
+    return out
+
+
 class Heap:
     """Indexed max-heap restatement (heap.hpp:20-94)."""
 
@@ -221,6 +234,7 @@ class Result:
     energies: np.ndarray    # [R][I+1]
     gaps: np.ndarray        # [R][I]
     status: int
+    followed: np.ndarray = None   # [R][I]: 1 where the loop took the followed entry
 
 
 @dataclass
@@ -237,8 +251,13 @@ def plant_narrow_dirty_rule(on):
 
 
 def decompose(d: Dictionary, signal, variant, iteration_max, energy_percent,
-              threads=0, iter_limit=None, check=True, trace=None) -> Result:
-    """trace: a list that receives one TraceStep per subtracting iteration."""
+              threads=0, iter_limit=None, check=True, trace=None, follow=None,
+              follow_tol=1e-4) -> Result:
+    """trace: a list that receives one TraceStep per subtracting iteration.
+    follow: [R][I][3] (k, f, t) per row (channels for SMP, else one row), -1
+    for none; where that entry differs from the oracle's own best and its
+    product is within follow_tol of the best, the loop takes it (a near-tie
+    branch) and Result.followed marks it."""
     sig = np.ascontiguousarray(np.atleast_2d(signal), dtype=np.float64)
     Cn, N = sig.shape
     assert N == d.N
@@ -259,10 +278,17 @@ def decompose(d: Dictionary, signal, variant, iteration_max, energy_percent,
         trace.append(TraceStep(iteration, row, res, maps))
 
     cb = _TRACE_FN(on_step) if trace is not None else _TRACE_FN()
-    status = lib().eo_decompose_traced(
+    followed = np.zeros(rows * I, dtype=np.int32)
+    fp = None
+    if follow is not None:
+        fa = np.ascontiguousarray(follow, dtype=np.int32)
+        assert fa.shape == (rows, I, 3), (fa.shape, rows, I)
+        fp = fa.ctypes.data_as(C.POINTER(C.c_int))
+    status = lib().eo_decompose_follow(
         C.byref(d._d), _dp(sig), Cn, C.byref(st), atoms,
         counts.ctypes.data_as(C.POINTER(C.c_int)), _dp(energies), _dp(gaps), int(threads),
-        int(iter_limit if iter_limit is not None else 2**31 - 1), cb, None)
+        int(iter_limit if iter_limit is not None else 2**31 - 1), cb, None, fp, float(follow_tol),
+        followed.ctypes.data_as(C.POINTER(C.c_int)))
     if check and status != 0:
         raise OracleError(status)
     params = np.zeros((Cn, I, 6))
@@ -273,4 +299,4 @@ def decompose(d: Dictionary, signal, variant, iteration_max, energy_percent,
             params[c, i] = list(a.params)
             index[c, i] = (a.k, a.f, a.t)
     return Result(params, index, counts, energies.reshape(rows, I + 1), gaps.reshape(rows, I),
-                  status)
+                  status, followed.reshape(rows, I))

@@ -9,7 +9,7 @@ import pytest
 
 import pyoracle as po
 from empi_amd import synth
-from parity import compared_iterations
+from parity import AMP_RTOL, oracle_following
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,37 @@ def _read_book(path):
     out = subprocess.run([TOOL, "readbook", str(path)], capture_output=True, text=True, check=True).stdout
     rows = [line.split() for line in out.splitlines() if not line.startswith(("fs", "epoch"))]
     return [(int(r[0]), np.array([float(v) for v in r[1:]])) for r in rows]
+
+
+def _book_index(d, params):
+    """(k, f, t) of book atoms from their float32 parameters (position and
+    scale in samples, frequency in units of Nyquist)."""
+    scales = np.array([sc.s * d.fs for sc in d.scales])
+    out = []
+    for p in params:
+        k = int(np.argmin(np.abs(scales - p[3])))
+        sc = d.scales[k]
+        out.append((k, int(round(p[4] * sc.Nfft / 2)), int(round(p[2] / d.fs * (sc.T - 1) / d.tMax))))
+    return np.array(out, dtype=np.int64)
+
+
+def _book_parity(d, x, variant, iters, percent, params, where):
+    """Book atoms (float32 parameters) against the oracle following their
+    indices: indices, counts, float32-rounded position, scale, frequency and
+    amplitude at every iteration."""
+    params = np.asarray(params, np.float64)
+    C = x.shape[0]
+    n = len(params) // C
+    per = params.reshape(C, n, 6)
+    index = np.stack([_book_index(d, per[c]) for c in range(C)])
+    ora = oracle_following(d, x, variant, iters, percent, index, [n] * C)
+    for c in range(C):
+        assert int(ora.counts[c]) == n, (where, c, int(ora.counts[c]), n)
+        assert np.array_equal(ora.index[c, :n], index[c]), (where, c)
+        want = ora.params[c, :n].astype(np.float32).astype(np.float64)
+        assert np.allclose(per[c, :, 2:5], want[:, 2:5], rtol=1e-6), (where, c)
+        assert np.all(np.abs(per[c, :, 1] / want[:, 1] - 1) < AMP_RTOL), (where, c)
+    return n
 
 
 @pytest.mark.parametrize("per_call", [False, True])
@@ -56,15 +87,10 @@ def test_cli_smp_two_epochs(tmp_path, per_call):
     for e in range(2):
         for c in range(2):
             x = sig[c, e * N:(e + 1) * N][None, :].astype(np.float64)
-            ora = po.decompose(d, x, po.SMP, 20, 99.0)
-            n = int(ora.counts[0])
-            got = [p for ch, p in atoms[k:k + n]]
-            assert all(ch == c + 1 for ch, _ in atoms[k:k + n])
-            m = compared_iterations(ora.gaps[0], n)
-            for i in range(m):
-                want = ora.params[0, i].astype(np.float32).astype(np.float64)
-                assert np.allclose(got[i][2:5], want[2:5], rtol=1e-6), (e, c, i)
-                assert abs(got[i][1] / want[1] - 1) < 1e-4
+            n = 0
+            while k + n < len(atoms) and atoms[k + n][0] == c + 1:
+                n += 1
+            _book_parity(d, x, po.SMP, 20, 99.0, [p for _, p in atoms[k:k + n]], f"epoch {e} channel {c}")
             k += n
             atom_lines = [ln for ln in lines if ln.startswith("ATOM\t")]
             assert atom_lines[0].split("\t")[:3] == ["ATOM", "0", str(d.entries)]
@@ -97,13 +123,10 @@ def test_cli_mmp_variants_name_and_run(tmp_path):
     assert (tmp_path / "sig_mmp.b").exists()
     assert "CHANNEL" not in r.stdout
     atoms = _read_book(tmp_path / "sig_mmp.b")
-    ora = po.decompose(po.Dictionary(128.0, 1280, 0.01), x.astype(np.float64), po.MMP3, 8, 99.0)
-    n = int(ora.counts[0])
-    assert len(atoms) == 4 * n
-    m = compared_iterations(ora.gaps[0], n)
-    for c in range(4):
-        for i in range(m):
-            assert np.allclose(atoms[c * n + i][1][2:5], ora.params[c, i, 2:5].astype(np.float32), rtol=1e-6)
+    assert len(atoms) % 4 == 0
+    assert [ch for ch, _ in atoms] == [c + 1 for c in range(4) for _ in range(len(atoms) // 4)]
+    _book_parity(po.Dictionary(128.0, 1280, 0.01), x.astype(np.float64), po.MMP3, 8, 99.0,
+                 [p for _, p in atoms], "MMP3 book")
 
 
 @pytest.mark.parametrize("per_batch", [2, 3])

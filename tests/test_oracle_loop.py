@@ -267,3 +267,50 @@ def test_planted_narrow_dirty_rule_is_caught(case):
             run_loop_checks(*case, rebuild=False)
     finally:
         po.plant_narrow_dirty_rule(False)
+
+
+@pytest.mark.parametrize("variant", [po.SMP, po.MMP1, po.MMP3])
+def test_energy_drop_identity(variant):
+    """fs (E_i - E_{i+1}) = sum_c (2 modulus^2 - ||w||^2) at every iteration,
+    edge-clipped atoms included: the identity the GPU tests assert for every
+    unit at full size."""
+    from parity import energy_drop_identity
+    if variant == po.SMP:
+        x = synth.batched_channels(items=1, N=2048, fs=256.0)[:1].astype(np.float64)
+        d = po.Dictionary(256.0, 2048, 0.01)
+    else:
+        x = synth.shared_multichannel(C=3, N=1280, fs=128.0, atoms=4, noise_sigma=0.1, seed=5).astype(np.float64)
+        d = po.Dictionary(128.0, 1280, 0.01)
+    r = po.decompose(d, x, variant, 25, 99.9)
+    n = int(r.counts[0])
+    assert energy_drop_identity(d.fs, d.N, r.params[:, :n], r.energies[0], n) < 1e-11
+
+
+def test_follow_takes_near_ties_only():
+    """eo_decompose_follow: on a time-symmetric signal the first atom's mirror
+    image (t -> T-1-t) has the same product to rounding, so following it is
+    taken, the original first atom comes second, and the residual energy after
+    both equals the unfollowed run's; an entry far below the best is not
+    followed."""
+    fs, N = 64.0, 511
+    t = np.arange(N) / fs
+    rng = np.random.default_rng(31)
+    x = 3.0 * np.exp(-np.pi * (t - 2.0) ** 2 / 0.5 ** 2) * np.cos(2 * np.pi * 9.0 * (t - 2.0) + 0.7)
+    x += 0.1 * rng.standard_normal(N)
+    x = x + x[::-1]                          # atoms at 2.0 s and tMax - 2.0 s
+    d = po.Dictionary(fs, N, 0.05)
+    r = po.decompose(d, x[None, :], po.SMP, 6, 100.0)
+    k, f, t = (int(v) for v in r.index[0, 0])
+    mirror = (k, f, d.scales[k].T - 1 - t)
+    assert mirror != (k, f, t)
+    follow = np.full((1, 6, 3), -1, np.int32)
+    follow[0, 0] = mirror
+    m = po.decompose(d, x[None, :], po.SMP, 6, 100.0, follow=follow)
+    assert m.followed[0, 0] == 1 and tuple(m.index[0, 0]) == mirror
+    assert tuple(r.index[0, 1]) == mirror and tuple(m.index[0, 1]) == (k, f, t)
+    assert m.followed[0, 1] == 0
+    assert abs(m.energies[0, 2] / r.energies[0, 2] - 1) < 1e-12
+    # a low entry (the DC bin of the smallest scale) is not taken
+    follow[0, 0] = (0, 0, 0)
+    low = po.decompose(d, x[None, :], po.SMP, 6, 100.0, follow=follow)
+    assert low.followed[0, 0] == 0 and tuple(low.index[0, 0]) == (k, f, t)
