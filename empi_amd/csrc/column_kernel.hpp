@@ -131,17 +131,22 @@ struct CellInfo {
 // MODE kModeSingle: one channel, no constraint (SMP units, MMP2 sums);
 // kModeSum: sum over channels of the products (MMP3); kModeMmp1: MMP1
 // constraint on low-frequency bins, channel sum on high ones.
+// LB (single-channel only, the proposal's evaluations): each re-evaluated
+// cell also writes celllow = the maximum over its bins 1 <= j < JB of the
+// product at E = 0, K' |X|^2, which bounds what a bin's product can become
+// when a subtraction turns its phase (the proposal's region bound).
 // Persistent: workgroups stride through the class's affected tiles.
-template <int Q, int MODE, int W>
+template <int Q, int MODE, int W, bool LB>
 __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W))) void k_columns(
     const ClassDev* __restrict__ classes, int cls, const ScaleDev* __restrict__ scales,
     const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
     const float* __restrict__ residual, unsigned long long* __restrict__ cellbest,
     unsigned long long* __restrict__ tilebest,
-    unsigned long long* __restrict__ counter) {
+    unsigned long long* __restrict__ counter, float* __restrict__ celllow) {
     using G_ = Geo<Q>;
     constexpr int NT = G_::NT, NPTS = G_::NPTS, NCELL = G_::NCELL, EPT = G_::EPT;
     constexpr bool MULTI = MODE != kModeSingle;
+    static_assert(!(LB && MULTI), "low-bin bests are single-channel only");
     constexpr int NACC = MULTI ? EPT : 1;
     constexpr int NACC3 = MODE == kModeMmp1 ? EPT : 1;
     constexpr int SEG = 2 * Q < 64 ? 2 * Q : 64;          // lanes sharing a cell
@@ -150,6 +155,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     __shared__ float2 buf[Lds<Q, NPTS>::SIZE];
     __shared__ CellInfo info[NCELL];
     __shared__ unsigned long long best[NCELL];
+    __shared__ unsigned lowbest[LB ? NCELL : 1];   // FP32 bits of a non-negative maximum
     __shared__ unsigned long long tbest;
     __shared__ unsigned cnt_lds;
     __shared__ float2 etl[kEtabLds + 1];    // ((1 - E)/K', 2E/K') of the item's scale; [kEtabLds] = (1/K', 0)
@@ -220,6 +226,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             }
             info[c] = ci;
             best[c] = 0ull;
+            if constexpr (LB) lowbest[c] = 0u;
         }
         if (tid == 0) {
             cnt_lds = 0;
@@ -315,7 +322,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             // the multichannel accumulators.
             // product of bin k from its spectrum value X = Xr + i Xi
             auto from_x = [&](const int k, const float Xr, const float Xi, const CellInfo& ci, const int i,
-                              unsigned long long& run) {
+                              unsigned long long& run, float& low) {
                 const bool high = 4 * k > Nfft;
                 const int j = high ? M - k : k;
                 const float num = Xr * Xr + Xi * Xi;
@@ -341,6 +348,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     if (j >= kEtabLds) tt = sd.etab[j];
                     const float den = fmaf(tt.x, num, tt.y * a * a);
                     pe = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
+                    if constexpr (LB)
+                        if (j >= 1 && j < sd.JB) low = fmaxf(low, num * sd.Kp);
                 }
                 if constexpr (!MULTI) {
                     ++mycnt;
@@ -356,7 +365,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 }
             };
             auto element = [&](const int rem, const int c, const CellInfo& ci, const int i,
-                               unsigned long long& run) {
+                               unsigned long long& run, float& low) {
                 const int half = rem / Q, k1 = rem % Q;
                 const int k2 = half ? ci.kb : ci.ka;
                 const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
@@ -371,7 +380,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const float2 wv = cis_rev(-static_cast<float>(k) * invN);
                 const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
                 const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
-                from_x(k, Xr, Xi, ci, i, run);
+                from_x(k, Xr, Xi, ci, i, run, low);
             };
             // Two bins from one pair of LDS values (single-channel units,
             // Q >= NT): bin k and its mirror M - k share A = Z[k] + conj Z[M-k],
@@ -380,7 +389,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             // with (slot b, Q-1-u) for g > 0; for g = 0, (a, u) with (a, Q-u) for
             // u < Q/2 and (b, u') with (b, Q-1-u'), u' = u - Q/2; u = 0 also takes
             // the self-mirrored (a, Q/2).  The Nyquist bin keeps its own path.
-            auto pair = [&](const int u, const int c, const CellInfo& ci, unsigned long long& run) {
+            auto pair = [&](const int u, const int c, const CellInfo& ci, unsigned long long& run, float& low) {
                 if (!ci.valid) return;
                 int sa, ka1, sb, kb1, k2;
                 if (ci.ka != 0) {
@@ -399,15 +408,15 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
                 const float2 wv = cis_rev(-static_cast<float>(k) * invN);
                 const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
-                if (k < F) from_x(k, 0.5f * (Ar + pi), 0.5f * (Ai - pr), ci, 0, run);
-                if (km < F && km != M) from_x(km, 0.5f * (Ar - pi), -0.5f * (Ai + pr), ci, 0, run);
+                if (k < F) from_x(k, 0.5f * (Ar + pi), 0.5f * (Ai - pr), ci, 0, run, low);
+                if (km < F && km != M) from_x(km, 0.5f * (Ar - pi), -0.5f * (Ai + pr), ci, 0, run, low);
                 if (ci.ka == 0 && u == 0 && M / 2 < F) {
                     // (a, Q/2): bin M/2, its own mirror (Z[k] = Z[M-k])
                     const float2 Zh = buf[lpad<Q>(2 * c * Q + Q / 2)];
                     const float2 wh = cis_rev(-static_cast<float>(M / 2) * invN);
                     const float hBi = Zh.y + Zh.y;                 // B = (0, 2 Im Zh), A = (2 Re Zh, 0)
                     const float hpr = -wh.y * hBi, hpi = wh.x * hBi;
-                    from_x(M / 2, 0.5f * (Zh.x + Zh.x + hpi), -0.5f * hpr, ci, 0, run);
+                    from_x(M / 2, 0.5f * (Zh.x + Zh.x + hpi), -0.5f * hpr, ci, 0, run, low);
                 }
             };
             constexpr bool CONTIG = !MULTI && Q >= 8 && Q <= 256;
@@ -420,11 +429,17 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const CellInfo ci = info[c];
                 const int r0 = (tid % TPC) * EPT;
                 unsigned long long run = 0ull;
+                float low = 0.0f;
 #pragma unroll 2
-                for (int s2 = 0; s2 < EPT; ++s2) element(r0 + ((s2 + tid) & (EPT - 1)), c, ci, 0, run);
+                for (int s2 = 0; s2 < EPT; ++s2) element(r0 + ((s2 + tid) & (EPT - 1)), c, ci, 0, run, low);
 #pragma unroll
                 for (int off = TPC / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
                 if ((tid & (TPC - 1)) == 0 && run) atomicMax(&best[c], run);
+                if constexpr (LB) {
+#pragma unroll
+                    for (int off = TPC / 2; off >= 1; off >>= 1) low = fmaxf(low, __shfl_xor(low, off));
+                    if ((tid & (TPC - 1)) == 0 && low > 0.0f) atomicMax(&lowbest[c], __float_as_uint(low));
+                }
             } else {
                 // elements of one group share a cell: its info is read once
 #pragma unroll MULTI ? EPT / GRP : 1
@@ -432,23 +447,29 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     const int c = (tid + gi * GRP * NT) / (2 * Q);
                     const CellInfo ci = info[c];
                     unsigned long long run = 0ull;
+                    float low = 0.0f;
                     if constexpr (MULTI) {
 #pragma unroll
                         for (int q = 0; q < GRP; ++q) {
                             const int i = gi * GRP + q;
-                            element((tid + i * NT) % (2 * Q), c, ci, i, run);
+                            element((tid + i * NT) % (2 * Q), c, ci, i, run, low);
                             __builtin_amdgcn_sched_barrier(0);   // keep per-element live ranges short
                         }
                     } else {
                         if constexpr (GRP >= 2) {
 #pragma unroll 2
-                            for (int q = 0; q < GRP / 2; ++q) pair(tid + q * NT, c, ci, run);
+                            for (int q = 0; q < GRP / 2; ++q) pair(tid + q * NT, c, ci, run, low);
                         } else {
-                            element((tid + gi * NT) % (2 * Q), c, ci, gi, run);
+                            element((tid + gi * NT) % (2 * Q), c, ci, gi, run, low);
                         }
 #pragma unroll
                         for (int off = SEG / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
                         if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
+                        if constexpr (LB) {
+#pragma unroll
+                            for (int off = SEG / 2; off >= 1; off >>= 1) low = fmaxf(low, __shfl_xor(low, off));
+                            if ((lane & (SEG - 1)) == 0 && low > 0.0f) atomicMax(&lowbest[c], __float_as_uint(low));
+                        }
                     }
                 }
             }
@@ -528,6 +549,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             if (ci.valid) {
                 cb = best[c];
                 cellbest[job.cell_off + cell0 + c] = cb;
+                if constexpr (LB) celllow[job.cell_off + cell0 + c] = __uint_as_float(lowbest[c]);
             } else {
                 cb = cellbest[job.cell_off + cell0 + c];
             }
@@ -551,29 +573,36 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     }
 }
 
-template <int Q, int MODE, int W>
+template <int Q, int MODE, int W, bool LB>
 void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
                     const UnitDev* units, const UnitState* state, const float* residual,
-                    unsigned long long* cellbest, unsigned long long* tilebest, unsigned long long* counter) {
-    hipLaunchKernelGGL((k_columns<Q, MODE, W>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
-                       state, residual, cellbest, tilebest, counter);
+                    unsigned long long* cellbest, unsigned long long* tilebest, unsigned long long* counter,
+                    float* celllow) {
+    hipLaunchKernelGGL((k_columns<Q, MODE, W, LB>), dim3(grid), dim3(Geo<Q>::NT), 0, s, classes, cls, scales, units,
+                       state, residual, cellbest, tilebest, counter, celllow);
 }
 
 // the launcher of class Q in a multichannel mode
 template <int Q>
 ColumnLauncher multichannel_launcher_for(int mode) {
-    if (mode == kModeSum) return launch_columns<Q, kModeSum, column_waves<Q, kModeSum>()>;
-    if (mode == kModeMmp1) return launch_columns<Q, kModeMmp1, column_waves<Q, kModeMmp1>()>;
+    if (mode == kModeSum) return launch_columns<Q, kModeSum, column_waves<Q, kModeSum>(), false>;
+    if (mode == kModeMmp1) return launch_columns<Q, kModeMmp1, column_waves<Q, kModeMmp1>(), false>;
     return nullptr;
 }
 
-// the launcher of class Q in one mode / wave variant
+// the launcher of class Q in one mode / wave variant; low: the single-channel
+// kernel that also writes the cells' low-bin bests
 template <int Q>
-ColumnLauncher launcher_for(int mode, int variant) {
-    if (mode != kModeSingle) return multichannel_launcher_for<Q>(mode);
-    if (variant == 1) return launch_columns<Q, kModeSingle, variant_waves<Q, 1>()>;
-    if (variant == 2) return launch_columns<Q, kModeSingle, variant_waves<Q, 2>()>;
-    return launch_columns<Q, kModeSingle, variant_waves<Q, 0>()>;
+ColumnLauncher launcher_for(int mode, int variant, bool low) {
+    if (mode != kModeSingle) return low ? nullptr : multichannel_launcher_for<Q>(mode);
+    if (low) {
+        if (variant == 1) return launch_columns<Q, kModeSingle, variant_waves<Q, 1>(), true>;
+        if (variant == 2) return launch_columns<Q, kModeSingle, variant_waves<Q, 2>(), true>;
+        return launch_columns<Q, kModeSingle, variant_waves<Q, 0>(), true>;
+    }
+    if (variant == 1) return launch_columns<Q, kModeSingle, variant_waves<Q, 1>(), false>;
+    if (variant == 2) return launch_columns<Q, kModeSingle, variant_waves<Q, 2>(), false>;
+    return launch_columns<Q, kModeSingle, variant_waves<Q, 0>(), false>;
 }
 
 }  // namespace empi

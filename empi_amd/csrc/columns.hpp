@@ -53,14 +53,15 @@ constexpr int variant_waves() { return V == 0 ? 4 : V == 1 ? 3 : 2; }
 typedef void (*ColumnLauncher)(int grid, hipStream_t s, const ClassDev* classes, int cls, const ScaleDev* scales,
                                const UnitDev* units, const UnitState* state, const float* residual,
                                unsigned long long* cellbest, unsigned long long* tilebest,
-                               unsigned long long* counter);
+                               unsigned long long* counter, float* celllow);
 
 // launcher of class lgQ (log2 Q) in mode `mode`; `variant` selects the wave
-// variant of the single-channel kernels; nullptr for a class that has no
-// kernel in that mode (Q = 8192 single-channel).  Q = 2..64 / 128..1024 /
-// 2048..8192.
-ColumnLauncher column_launcher_small(int lgQ, int mode, int variant);
-ColumnLauncher column_launcher_mid(int lgQ, int mode, int variant);
-ColumnLauncher column_launcher_large(int lgQ, int mode, int variant);
+// variant of the single-channel kernels, `low` the single-channel kernel that
+// also writes the cells' low-bin bests (celllow); nullptr for a class that
+// has no kernel in that mode (Q = 8192 single-channel).  Q = 2..64 /
+// 128..1024 / 2048..8192.
+ColumnLauncher column_launcher_small(int lgQ, int mode, int variant, bool low);
+ColumnLauncher column_launcher_mid(int lgQ, int mode, int variant, bool low);
+ColumnLauncher column_launcher_large(int lgQ, int mode, int variant, bool low);
 
 }  // namespace empi

@@ -3,12 +3,12 @@
 
 namespace empi {
 
-ColumnLauncher column_launcher_mid(int lgQ, int mode, int variant) {
+ColumnLauncher column_launcher_mid(int lgQ, int mode, int variant, bool low) {
     switch (lgQ) {
-    case 7: return launcher_for<128>(mode, variant);
-    case 8: return launcher_for<256>(mode, variant);
-    case 9: return launcher_for<512>(mode, variant);
-    case 10: return launcher_for<1024>(mode, variant);
+    case 7: return launcher_for<128>(mode, variant, low);
+    case 8: return launcher_for<256>(mode, variant, low);
+    case 9: return launcher_for<512>(mode, variant, low);
+    case 10: return launcher_for<1024>(mode, variant, low);
     default: return nullptr;
     }
 }

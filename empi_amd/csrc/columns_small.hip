@@ -3,14 +3,14 @@
 
 namespace empi {
 
-ColumnLauncher column_launcher_small(int lgQ, int mode, int variant) {
+ColumnLauncher column_launcher_small(int lgQ, int mode, int variant, bool low) {
     switch (lgQ) {
-    case 1: return launcher_for<2>(mode, variant);
-    case 2: return launcher_for<4>(mode, variant);
-    case 3: return launcher_for<8>(mode, variant);
-    case 4: return launcher_for<16>(mode, variant);
-    case 5: return launcher_for<32>(mode, variant);
-    case 6: return launcher_for<64>(mode, variant);
+    case 1: return launcher_for<2>(mode, variant, low);
+    case 2: return launcher_for<4>(mode, variant, low);
+    case 3: return launcher_for<8>(mode, variant, low);
+    case 4: return launcher_for<16>(mode, variant, low);
+    case 5: return launcher_for<32>(mode, variant, low);
+    case 6: return launcher_for<64>(mode, variant, low);
     default: return nullptr;
     }
 }

@@ -346,7 +346,10 @@ struct RegionState {
     int flags;                       // bit 0 dirty, bit 1 evaluate now
     unsigned long long rbest;        // packed (product bits, ~bin) of the best cell
     int rcell;                       // best cell index within the scale
-    float racc;                      // sqrt-product change bound accumulated since the evaluation
+    float racc;                      // bound on |du| accumulated since the evaluation, all bins
+    float rlow;                      // max over bins 1 <= j < JB of |u|^2 (the product at E = 0) at the evaluation
+    float racc_lo;                   // bound on |du| accumulated since the evaluation, bins 1 <= j < JB
+    int pad;
 };
 
 struct SelBest {
@@ -507,12 +510,14 @@ __device__ inline bool atom_range(const ScaleDev& sd, const UnitState& st, int* 
     return l <= h;
 }
 
-// region bests from the cell bests: all regions, or those flagged "evaluate"; one wave per region
+// region bests from the cell bests (and the low-bin bound seeds from the
+// cells' low-bin bests): all regions, or those flagged "evaluate"; one wave per region
 __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev* __restrict__ units,
                                                                    const UnitState* __restrict__ state,
                                                                    const ScaleDev* __restrict__ scales,
                                                                    const RegionDev* __restrict__ regtab,
                                                                    const unsigned long long* __restrict__ cellbest,
+                                                                   const float* __restrict__ celllow,
                                                                    RegionState* __restrict__ regions, int all) {
     const int u = blockIdx.y;
     if (!all && !state[u].active) return;
@@ -528,6 +533,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev
         const ScaleDev& sd = scales[ud.scale0 + l];
         unsigned long long best = 0ull;
         int bcell = 0x7FFFFFFF;
+        float low = 0.0f;
         if (rt.tlo <= rt.thi) {
             const long long base = ud.cell_off + sd.cells_before;
             const int c0 = rt.tlo * sd.G, c1 = rt.thi * sd.G + sd.G - 1;
@@ -537,6 +543,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev
                     best = v;
                     bcell = c;
                 }
+                low = fmaxf(low, celllow[base + c]);
             }
         }
         for (int off = 32; off >= 1; off >>= 1) {
@@ -546,6 +553,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev
                 best = ov;
                 bcell = oc;
             }
+            low = fmaxf(low, __shfl_xor(low, off));
         }
         if (lane == 0) {
             R->rbest = best;
@@ -553,23 +561,33 @@ __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev
             R->rcell = bcell;
             R->flags = 0;
             R->racc = 0.0f;
+            R->racc_lo = 0.0f;
+            R->rlow = low;
         }
     }
 }
 
 // Regions the unit's last atom A overlaps (the reference's rule) become
-// dirty, and each accumulates a bound on how far A's subtraction can raise
-// the square root of any of its products: |<R - a g_A, g>| <= |<R, g>| +
-// |a| |<g_A, g>| for every unit real atom g of the region, with (continuous
-// Gabor atoms, n = the real-atom norms, I = the envelope overlap)
-//   |<g_A, g>| <= n_A n / 2 * I * (1 + e^{-pi fA^2 s^2 sA^2 / (s^2 + sA^2)}),
-//   I = s sA / sqrt(s^2 + sA^2) * e^{-pi dt^2 / (s^2 + sA^2)},
-// dt the distance from tA to the region's column centres, n = sqrt(2 sqrt 2 / s)
-// away from the lowest bins; 1% covers the discrete sums, and an additive
-// 2 (e^{-pi (s fs)^2 / 8} + e^{-pi (sA fs)^2 / 8}) their aliasing at scales of
-// a few samples.  The first bins, whose product normalisation depends on the
-// phase, can exceed this bound when a subtraction turns their phase; the
-// Cauchy-Schwarz bound holds there.
+// dirty, and each accumulates a bound r on how far the subtractions since its
+// evaluation move u = sqrt(K) v, v the map value (linear in the residual):
+// the product is P = |u|^2 h(phi)^2, h = (1 + E cos 2 phi)^(-1/2)
+// (gabor.cpp:62-70), so for bin j either
+//   (A) sqrt P_new <= (|u_old| + r) / sqrt(1 - E_j), or
+//   (B) sqrt P_new <= sqrt P_old + L(E_j) r, L the Lipschitz constant of |u| h
+// (k_rows takes the smaller over bins 1 <= j < JB, (B) elsewhere; bins 0 and
+// M are real, with a fixed denominator).  r = |a| sqrt(K) |v(g_A)| summed over
+// the atoms, g_A the unit atom; by Poisson summation the discrete overlap of
+// two sampled Gabor atoms is
+//   sqrt(K) |v(g_A)| <= n_A n I / 2 (PG(fA - f) + PG(fA + f)),
+//   I = s sA / sqrt(s^2 + sA^2) e^{-pi dt^2 / (s^2 + sA^2)},
+//   PG(x) = sum_m e^{-pi sg^2 (x - m fs)^2}, sg^2 = s^2 sA^2 / (s^2 + sA^2),
+// dt the distance from tA to the region's column centres, n = sqrt(2 sqrt 2 / s),
+// n_A the atom's own norm.  racc takes the maximum over all bins f, racc_lo
+// over the low bins, which lie within f_L = (JB - 1) fs / Nfft of 0 and of
+// fs / 2: an atom far from both in frequency barely moves them.  The
+// frequency decay needs whole Gaussians: where the atom's support or a window
+// of the region is clipped by the signal's ends only the envelope bound
+// n_A n I PG(0) holds.  1% and 1e-9 cover rounding and the 3 s truncation.
 __global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __restrict__ units,
                                                                const UnitState* __restrict__ state,
                                                                const ScaleDev* __restrict__ scales,
@@ -598,12 +616,31 @@ __global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __
         const double dt = st.tA < t0 ? t0 - st.tA : (st.tA > t1 ? st.tA - t1 : 0.0);
         const double v = s * s + sA * sA;
         const double overlap = s * sA / sqrt(v) * exp(-M_PI * dt * dt / v);
-        const double mirror = exp(-M_PI * fA * fA * s * s * sA * sA / v);
-        const double n = sqrt(2.0 * M_SQRT2 / s);
-        const double sfs = s * ud.fs, afs = sA * ud.fs;
-        const double alias = 2.0 * (exp(-M_PI * sfs * sfs / 8.0) + exp(-M_PI * afs * afs / 8.0));
-        const double rho = fmin(1.0, 1.01 * 0.5 * nA * n * overlap * (1.0 + mirror) + alias);
-        R->racc += static_cast<float>(coef * rho);
+        const bool clipped = lrint((t0 - kGaussHalfWidth * s) * ud.fs) < 0 ||
+                             lrint((t1 + kGaussHalfWidth * s) * ud.fs) > ud.N - 1 ||
+                             lrint((st.tA - kGaussHalfWidth * sA) * ud.fs) < 0 ||
+                             lrint((st.tA + kGaussHalfWidth * sA) * ud.fs) > ud.N - 1;
+        const double sg2 = s * s * sA * sA / v;
+        const double fs = ud.fs, fN = 0.5 * fs;
+        // sum_m e^{-pi sg2 (x - m fs)^2}; sg fs >= sqrt 2, so |m| <= 3 suffices
+        auto pg = [&](double x) {
+            double acc = 0.0;
+            for (int m = -3; m <= 3; ++m) acc += exp(-M_PI * sg2 * (x - m * fs) * (x - m * fs));
+            return acc;
+        };
+        const double base = 1.01 * nA * sqrt(2.0 * M_SQRT2 / s) * overlap;
+        double rho_hi, rho_lo;
+        if (clipped) {
+            rho_hi = rho_lo = base * pg(0.0);
+        } else {
+            const double fL = max(sd.JB - 1, 0) * fs / sd.Nfft;
+            rho_hi = base * 0.5 * (pg(0.0) + pg(fmin(fA, fN - fA)));
+            const double d_diff = fmin(fmax(0.0, fA - fL), fmax(0.0, fN - fL - fA));
+            const double d_sum = fmax(0.0, fmin(fmin(fA, fs - fA - fL), fmin(fN - fA, fN - fL + fA)));
+            rho_lo = base * 0.5 * (pg(d_diff) + pg(d_sum));
+        }
+        R->racc += static_cast<float>(coef * fmin(1.0, rho_hi + 1e-9));
+        R->racc_lo += static_cast<float>(coef * fmin(1.0, rho_lo + 1e-9));
     }
 }
 
@@ -750,8 +787,13 @@ __global__ __launch_bounds__(kRegionThreads) void k_rows(
         const long l0 = lrint((t0 - hw) * sd.fs), r1 = lrint((t1 + hw) * sd.fs);
         const int iL = static_cast<int>(l0 > 0 ? l0 : 0), iR = static_cast<int>(r1 < N - 1 ? r1 : N - 1);
         const float cs = sd.csnorm * static_cast<float>(pre[iR + 1] - pre[iL]);
-        const float grow = sqrtf(R->rmax) + R->racc;
-        const float bound = fminf(cs, grow * grow * 1.0001f);
+        // bins j >= JB and bins 0, M: (B) at E <= 1e-4 or a real value;
+        // bins 1 <= j < JB: the smaller of (A) from their |u| seeds and (B)
+        const float hi = sqrtf(R->rmax) + R->racc * sd.gH;
+        const float loA = (sqrtf(R->rlow) + R->racc_lo) * sd.gL;
+        const float loB = sqrtf(R->rmax) + R->racc_lo * sd.lL;
+        const float lo = fminf(loA, loB);
+        const float bound = fminf(cs, fmaxf(hi * hi, lo * lo) * 1.0001f);
         R->rbound = bound;
         const bool candidate = bound > limit;
         R->flags = candidate ? (R->flags | 4) : (R->flags & ~4);
@@ -1116,10 +1158,10 @@ int column_grid(int Q, long long tiles, int gv) {
     return static_cast<int>((g + 7) / 8 * 8);
 }
 
-ColumnLauncher column_launcher(int mode, int cls, int choice) {
+ColumnLauncher column_launcher(int mode, int cls, int choice, bool low = false) {
     const int lg = cls + kMinLgQ, v = choice % kWaveVariants;
-    return lg <= 6 ? column_launcher_small(lg, mode, v)
-         : lg <= 10 ? column_launcher_mid(lg, mode, v) : column_launcher_large(lg, mode, v);
+    return lg <= 6 ? column_launcher_small(lg, mode, v, low)
+         : lg <= 10 ? column_launcher_mid(lg, mode, v, low) : column_launcher_large(lg, mode, v, low);
 }
 
 // default tuning: the choice (wave variant + 3 * grid variant) per class that
@@ -1172,6 +1214,7 @@ struct empi_plan {
     int max_rows = 0;
     DevBuf<RegionDev> d_regtab;
     DevBuf<RegionState> d_regions;
+    DevBuf<float> d_celllow;              // per cell: max over bins 1 <= j < JB of P (1 + E)/(1 - E)
     DevBuf<double> d_prefix;
     DevBuf<long long> d_prefix_off;
     DevBuf<float> d_blockfeat;
@@ -1399,6 +1442,22 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
                 }
             }
             s.gA = static_cast<float>(M_PI / ((sc.s * d.fs) * (sc.s * d.fs)));
+            {
+                // the proposal's region bound: bins with E_j > 1e-4 take the low-bin seeds
+                const double alpha = sc.s * d.fs / sc.Nfft;
+                const int M = sc.Nfft / 2;
+                const double jb = std::ceil(std::sqrt(std::log(1e4) / (2.0 * M_PI)) / alpha);
+                s.JB = static_cast<int>(std::min<double>(M + 1, std::min<double>(jb, s.JT)));
+                const double eb = std::exp(-2.0 * M_PI * (s.JB * alpha) * (s.JB * alpha));
+                const double e1 = M >= 2 ? std::exp(-2.0 * M_PI * alpha * alpha) : 0.0;
+                // max over phi of |grad(|u| h)| = sqrt((1 + E^2 + 2 E c) / (1 + E c)^3), c = cos 2 phi
+                auto lip = [](double E) {
+                    return E >= 1.0 / 3.0 ? std::sqrt(32.0 / 27.0) / (1.0 - E * E) : 1.0 / std::sqrt(1.0 - E);
+                };
+                s.gH = static_cast<float>(lip(eb) * (1.0 + 1e-6));
+                s.lL = static_cast<float>(lip(e1) * (1.0 + 1e-6));
+                s.gL = static_cast<float>(1.0 / std::sqrt(1.0 - e1) * (1.0 + 1e-6));
+            }
             s.Wmax = sc.Wmax;
             s.fold = sc.fold;
             s.csnorm = static_cast<float>(proposal_bound_factor(sc.s, d.fs, sc.Nfft));
@@ -1547,10 +1606,12 @@ empi_status launch_class_columns(empi_plan* p, hipStream_t s) {
             EMPI_HIP(hipEventRecord(evp->first, s));
         }
         const int choice = p->tune[mode][c];
-        column_launcher(mode, c, choice)(column_grid(1 << (c + kMinLgQ), cj.tiles, choice / kWaveVariants), s,
-                                         p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p, p->d_state.p,
-                                         p->d_res32.p, p->d_cells.p, p->d_tiles.p,
-                                         p->d_counter.p);
+        // in proposal mode the single-channel kernels also write the cells' low-bin bests
+        const bool low = p->prop_mode != EMPI_PROPOSAL_OFF && mode == kModeSingle;
+        column_launcher(mode, c, choice, low)(column_grid(1 << (c + kMinLgQ), cj.tiles, choice / kWaveVariants), s,
+                                              p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p, p->d_state.p,
+                                              p->d_res32.p, p->d_cells.p, p->d_tiles.p,
+                                              p->d_counter.p, low ? p->d_celllow.p : nullptr);
         if (evp) EMPI_HIP(hipEventRecord(evp->second, s));
     }
     EMPI_HIP(hipGetLastError());
@@ -1591,6 +1652,7 @@ empi_status proposal_prepare(empi_plan* p) {
     empi_status st;
     hipStream_t s0 = nullptr;
     if ((st = p->d_regions.alloc(static_cast<size_t>(p->nregions))) != EMPI_OK) return st;
+    if ((st = p->d_celllow.alloc(p->d_cells.n)) != EMPI_OK) return st;
     if ((st = p->d_prefix.alloc(static_cast<size_t>(pre))) != EMPI_OK) return st;
     if ((st = p->d_prefix_off.alloc(U)) != EMPI_OK) return st;
     if ((st = p->d_blockfeat.alloc(static_cast<size_t>(feat))) != EMPI_OK) return st;
@@ -1621,7 +1683,7 @@ empi_status proposal_pass(empi_plan* p, hipStream_t s, unsigned long long* stat)
     empi_status st = launch_class_columns(p, s);
     if (st != EMPI_OK) return st;
     hipLaunchKernelGGL(k_regions_reduce, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
-                       p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_regions.p, 0);
+                       p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_celllow.p, p->d_regions.p, 0);
     EMPI_HIP(hipGetLastError());
     return EMPI_OK;
 }
@@ -1735,7 +1797,7 @@ empi_status plan_autotune(empi_plan* p, const float* signal, int on_device, int 
                     column_launcher(mode, c, choice)(column_grid(1 << (c + kMinLgQ), cj.tiles, choice / kWaveVariants),
                                                      s, p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p,
                                                      p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p,
-                                                     nullptr);
+                                                     nullptr, nullptr);
                     float ms = 0.0f;
                     if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
@@ -1789,7 +1851,7 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     if (prop) {
         EMPI_HIP(hipMemsetAsync(p->d_pstats.p, 0, sizeof(unsigned long long) * 5, s));
         hipLaunchKernelGGL(k_regions_reduce, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p,
-                           p->d_state.p, p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_regions.p, 1);
+                           p->d_state.p, p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_celllow.p, p->d_regions.p, 1);
         EMPI_HIP(hipGetLastError());
         if ((st = launch_features(p, s, 1)) != EMPI_OK) return st;
         if (graph && !p->prop_exec) {

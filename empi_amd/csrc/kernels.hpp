@@ -34,6 +34,11 @@ struct ScaleDev {
     int JT;                   // bins j < JT carry E = exp(-2 pi (j alpha)^2) > 0 in FP32
     const float2* etab;       // [JT]: ((1 - E)/K', 2E/K') in FP32 from FP64
     float csnorm;             // max discrete/continuous atom-norm ratio: P <= csnorm * window energy
+    // proposal bound: bins j >= JB have E_j <= 1e-4; L(E) the Lipschitz
+    // constant of sqrt(P) in |u| = sqrt(K') |X| at window term E:
+    // gH = L(E_JB), lL = L(E_1), gL = 1/sqrt(1 - E_1) (the largest over 1 <= j < JB)
+    int JB;
+    float gH, gL, lL;
 };
 
 struct UnitDev {

@@ -203,12 +203,15 @@ empi_status empi_mlp_train(empi_mlp* mlp, const float* X, const float* target, i
  * Each iteration then evaluates exactly: the top_k dirty candidate regions by
  * network score (by their last exact maximum when net is NULL), and, as the
  * full-search fallback, every dirty region whose certified bound exceeds
- * P / (1 - delta), P the best evaluated product.  The bound is the smaller of
- * csnorm * (residual energy under the region's windows) and the atom-overlap
- * bound (sqrt(last exact maximum) + sum over the atoms since of
- * |amplitude| * overlap)^2.  The chosen atom is therefore within (1 - delta) of
- * the exact maximum.  Single-channel units only (SMP, MMP2 sums, one-channel
- * items); EMPI_UNSUPPORTED otherwise.
+ * P / (1 - delta), P the best evaluated product.  The bound holds for every
+ * bin: it is the smaller of csnorm * (residual energy under the region's
+ * windows) and a bound from the region's last exact maximum grown by the
+ * atoms subtracted since (their discrete overlap with the region's atoms, by
+ * Poisson summation, times the Lipschitz constant of the reference's product
+ * in the map value, which is large at the first bins of each scale where the
+ * product's normalisation depends on the phase).  The chosen atom is
+ * therefore within (1 - delta) of the exact maximum.  Single-channel units
+ * only (SMP, MMP2 sums, one-channel items); EMPI_UNSUPPORTED otherwise.
  *   mode EMPI_PROPOSAL_OFF      the plain method (default)
  *   mode EMPI_PROPOSAL_ON       as above
  *   mode EMPI_PROPOSAL_COLLECT  every dirty region is evaluated (the plain

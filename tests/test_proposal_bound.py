@@ -82,41 +82,66 @@ def test_bound_tight_on_matched_atoms(fs, N):
 # ------------------------------------------------------------ region bound
 #
 # The proposal's bound on a dirty region's new maximum, restated in FP64 from
-# the device code (k_mark_dirty, k_rows): for every bin j >= 1 of every cell,
-# with P = K |v|^2 / (1 + E_j cos 2 phi) (gabor.cpp:62-70), E_j = exp(-2 pi (j
-# alpha)^2) and j counted from the nearer end of the spectrum,
-#   sqrt(P_new) <= sqrt(P_old (1 + E_j) / (1 - E_j)) + sum |a| rho / sqrt(1 - E_j),
-# since 1 - E <= 1 + E cos 2 phi <= 1 + E and sqrt(K)|v| is a seminorm of the
-# residual; bins 0 and M are real-valued with a fixed denominator.  Bins with
-# E_j > 1e-4 (j < JB) use a per-cell maximum of P (1 + E)/(1 - E) and the
-# factor of bin 1; the others the region maximum.
+# the device code (k_mark_dirty, k_rows).  A product is P = |u|^2 h(phi)^2,
+# u = sqrt(K) v linear in the residual, h = (1 + E_j cos 2 phi)^(-1/2)
+# (gabor.cpp:62-70), E_j = exp(-2 pi (j alpha)^2), j counted from the nearer
+# end of the spectrum.  For a change |du| <= r of u (r = |a| times the atom
+# overlap):
+#   (A) sqrt P_new <= (|u_old| + r) / sqrt(1 - E_j)       (1 + E cos 2 phi >= 1 - E)
+#   (B) sqrt P_new <= sqrt P_old + L(E_j) r,  L(E) = max |grad (|u| h)|
+#       = sqrt(32/27) / (1 - E^2) for E >= 1/3, else 1 / sqrt(1 - E)
+# Bins with E_j <= 1e-4 (j >= JB, and the real bins 0 and M) use (B) with
+# L(E_JB); bins 1 <= j < JB the smaller of (A) with a per-cell maximum of
+# |u|^2 and (B) with L(E_1).  (B) is exact when nothing changed.
 
 def e_of(alpha, j):
     return np.exp(-2 * np.pi * (j * alpha) ** 2)
 
 
+def lipschitz(E):
+    return np.sqrt(32.0 / 27.0) / (1 - E * E) if E >= 1.0 / 3.0 else 1.0 / np.sqrt(1 - E)
+
+
 def scale_factors(s, fs, Nfft):
-    """(JB, fH, gH, gL) of a scale, as the engine's plan sets them."""
+    """(JB, gH, gL, lL) of a scale, as the engine's plan sets them."""
     alpha = s * fs / Nfft
     M = Nfft // 2
     JB = int(min(M + 1, np.ceil(np.sqrt(np.log(1e4) / (2 * np.pi)) / alpha)))
     eb = e_of(alpha, JB)
     e1 = e_of(alpha, 1) if M >= 2 else 0.0
-    return JB, (1 + eb) / (1 - eb), 1 / np.sqrt(1 - eb), 1 / np.sqrt(1 - e1)
+    return JB, lipschitz(eb), 1 / np.sqrt(1 - e1), lipschitz(e1)
 
 
-def overlap_rho(s, fs, t0, t1, params, clipped):
-    """Bound on the overlap of the unit atom `params` with any unit real atom
-    of scale s and time in [t0, t1] (k_mark_dirty), at E = 0."""
+def _pg(sig2, fs, d):
+    """sum_m exp(-pi sig2 (d - m fs)^2): the sampled Gaussian's periodic spectrum."""
+    return sum(np.exp(-np.pi * sig2 * (d - m * fs) ** 2) for m in range(-3, 4))
+
+
+def overlap_rho(s, fs, Nfft, JB, t0, t1, params, clipped):
+    """(all bins, bins 1 <= j < JB) bounds on |u| of the unit atom `params`
+    at scale s, time in [t0, t1] (k_mark_dirty).  The discrete overlap of two
+    Gabor atoms is n_A n I / 2 |PG(fA - f) + PG(fA + f)| (Poisson summation,
+    I the envelope integral); the low bins lie within f_L of 0 and of fs/2;
+    where a window or the atom is clipped by the signal's ends only the
+    envelope bound n_A n I PG(0) holds."""
     modulus, amp, sA, fA, tA = params[0], params[1], params[3] / fs, params[4] * fs / 2, params[2] / fs
     nA = np.sqrt(fs) * abs(amp) / abs(modulus)
     dt = t0 - tA if tA < t0 else (tA - t1 if tA > t1 else 0.0)
     v = s * s + sA * sA
-    overlap = s * sA / np.sqrt(v) * np.exp(-np.pi * dt * dt / v)
-    mirror = 1.0 if clipped else np.exp(-np.pi * fA * fA * s * s * sA * sA / v)
+    sig2 = s * s * sA * sA / v
+    I = s * sA / np.sqrt(v) * np.exp(-np.pi * dt * dt / v)
     n = np.sqrt(2 * np.sqrt(2) / s)
-    alias = 2 * (np.exp(-np.pi * (s * fs) ** 2 / 8) + np.exp(-np.pi * (sA * fs) ** 2 / 8))
-    return min(1.0, 1.01 * 0.5 * nA * n * overlap * (1 + mirror) + alias)
+    base = 1.01 * nA * n * I
+    if clipped:
+        r = min(1.0, base * _pg(sig2, fs, 0.0) + 1e-9)
+        return r, r
+    fN = fs / 2
+    fL = max(JB - 1, 0) * fs / Nfft
+    hi = base * 0.5 * (_pg(sig2, fs, 0.0) + _pg(sig2, fs, min(fA, fN - fA)))
+    d_diff = min(max(0.0, fA - fL), max(0.0, fN - fL - fA))
+    d_sum = min(fA, fs - fA - fL, fN - fA, fN - fL + fA)
+    lo = base * 0.5 * (_pg(sig2, fs, d_diff) + _pg(sig2, fs, max(0.0, d_sum)))
+    return min(1.0, hi + 1e-9), min(1.0, lo + 1e-9)
 
 
 def region_columns(d, k, b):
@@ -135,109 +160,126 @@ def _clipped(d, k, tlo, thi, params):
             np.rint((tA - 3 * sA) * d.fs) < 0 or np.rint((tA + 3 * sA) * d.fs) > d.N - 1)
 
 
-def low_inflated(d, k, P):
-    """Per column: max over bins 1 <= j < JB of P (1 + E_j) / (1 - E_j)."""
+def low_seeds(d, k, x):
+    """Per column: max over bins 1 <= j < JB of |u|^2 = K |v|^2, the product at
+    E = 0 (the kernel's num * K')."""
     sc = d.scales[k]
-    alpha = sc.s * d.fs / sc.Nfft
     M = sc.Nfft // 2
     JB = scale_factors(sc.s, d.fs, sc.Nfft)[0]
     kb = np.arange(sc.F)
     j = np.where(4 * kb > sc.Nfft, M - kb, kb)
     low = (j >= 1) & (j < JB)
-    if not low.any():
-        return np.zeros(P.shape[1])
-    E = e_of(alpha, j[low])
-    return np.max(P[low] * ((1 + E) / (1 - E))[:, None], axis=0)
+    out = np.zeros(sc.T)
+    if low.any():
+        K = d.fs * 2 * np.sqrt(2) / sc.s
+        for t in range(sc.T):
+            out[t] = K * float(np.max(np.abs(d.column(k, x, t)[low]) ** 2))
+    return out
 
 
-def region_bounds(d, k, tlo, thi, P_old, atoms):
-    """(old bound, new bound) on the region's new maximum without the
-    Cauchy-Schwarz cap: P_old [F][T] the products at the last evaluation,
-    atoms the parameters subtracted since."""
+def region_bounds(d, k, tlo, thi, P_old, seeds, atoms):
+    """(bound without the E factors, bound) on the region's new maximum,
+    without the Cauchy-Schwarz cap: P_old [F][T] and seeds [T] at the last
+    evaluation, atoms the parameters subtracted since."""
     sc = d.scales[k]
     t0, t1 = tlo * d.tMax / (sc.T - 1), thi * d.tMax / (sc.T - 1)
     rmax = float(np.max(P_old[:, tlo:thi + 1]))
-    rlow = float(np.max(low_inflated(d, k, P_old)[tlo:thi + 1]))
-    racc = sum(abs(p[0]) * overlap_rho(sc.s, d.fs, t0, t1, p, _clipped(d, k, tlo, thi, p)) for p in atoms)
-    racc_old = sum(abs(p[0]) * overlap_rho(sc.s, d.fs, t0, t1, p, False) for p in atoms)
-    _, fH, gH, gL = scale_factors(sc.s, d.fs, sc.Nfft)
-    old = (np.sqrt(rmax) + racc_old) ** 2 * 1.0001
-    new = max((np.sqrt(rmax * fH) + racc * gH) ** 2, (np.sqrt(rlow) + racc * gL) ** 2) * 1.0001
-    return old, new
+    seed = float(np.max(seeds[tlo:thi + 1]))
+    JB, gH, gL, lL = scale_factors(sc.s, d.fs, sc.Nfft)
+    rh = [overlap_rho(sc.s, d.fs, sc.Nfft, JB, t0, t1, p, _clipped(d, k, tlo, thi, p)) for p in atoms]
+    racc_hi = sum(abs(p[0]) * r[0] for p, r in zip(atoms, rh))
+    racc_lo = sum(abs(p[0]) * r[1] for p, r in zip(atoms, rh))
+    old = (np.sqrt(rmax) + sum(abs(p[0]) * _old_rho(sc.s, d.fs, t0, t1, p) for p in atoms)) ** 2 * 1.0001
+    hi = (np.sqrt(rmax) + racc_hi * gH) ** 2
+    lo = min((np.sqrt(seed) + racc_lo) ** 2 * gL * gL, (np.sqrt(rmax) + racc_lo * lL) ** 2)
+    return old, max(hi, lo) * 1.0001
+
+
+def _old_rho(s, fs, t0, t1, params):
+    """The overlap bound the proposal used before the E factors and the
+    clipping rule (no counterpart in the device code now)."""
+    modulus, amp, sA, fA, tA = params[0], params[1], params[3] / fs, params[4] * fs / 2, params[2] / fs
+    nA = np.sqrt(fs) * abs(amp) / abs(modulus)
+    dt = t0 - tA if tA < t0 else (tA - t1 if tA > t1 else 0.0)
+    v = s * s + sA * sA
+    overlap = s * sA / np.sqrt(v) * np.exp(-np.pi * dt * dt / v)
+    mirror = np.exp(-np.pi * fA * fA * s * s * sA * sA / v)
+    n = np.sqrt(2 * np.sqrt(2) / s)
+    alias = 2 * (np.exp(-np.pi * (s * fs) ** 2 / 8) + np.exp(-np.pi * (sA * fs) ** 2 / 8))
+    return min(1.0, 1.01 * 0.5 * nA * n * overlap * (1 + mirror) + alias)
 
 
 def _atom(d, k, f, t, amp, phase):
-    """A dictionary atom's waveform and its book parameters (its own map value)."""
+    """An atom as the loop subtracts it: parameters from the map value of a
+    dictionary waveform of amplitude amp and phase `phase` at (k, f, t)
+    (gabor.cpp:111-131), and the waveform those parameters give
+    (gabor.cpp:143-187; clipped at the edges its amplitude is the map's)."""
     sc = d.scales[k]
     t0 = t * d.tMax / (sc.T - 1)
     w, _ = po.subtract(d.fs, [0.0, amp, t0 * d.fs, sc.s * d.fs, 2.0 * f / sc.Nfft, phase], np.zeros(d.N))
-    w = -w
-    v = d.column(k, w, t)[f]
-    return w, d.atom_params(k, f, t, np.array([v]))[0]
+    v = d.column(k, -w, t)[f]
+    p = d.atom_params(k, f, t, np.array([v]))[0]
+    w, _ = po.subtract(d.fs, p, np.zeros(d.N))
+    return -w, p
 
 
-def test_region_bound_covers_low_bin_phase_turn():
-    """A subtraction that turns a low bin's phase from cosine to sine raises its
-    product by up to (1 + E)/(1 - E) at unchanged |v|: the cell holds a
-    cosine-phase atom A plus a sine-phase atom of 3x the amplitude at bin 1.
-    The overlap bound without the E factors misses the new maximum; the
-    bound with them holds."""
-    fs, N = 128.0, 1280
-    d = po.Dictionary(fs, N, 0.01)
-    k = next(i for i, sc in enumerate(d.scales) if sc.s * fs >= 20 and sc.T > 8)
-    sc = d.scales[k]
-    t = sc.T // 2
-    A, pA = _atom(d, k, 1, t, 1.0, 0.0)
-    S, _ = _atom(d, k, 1, t, 3.0, np.pi / 2)
-    x = A + S
-    b = int(np.rint(t * (N - 1) / (sc.T - 1))) // 256
-    tlo, thi = region_columns(d, k, b)
-    assert tlo <= t <= thi
-    P_old = d.fresh_products(k, x)
-    P_new = d.fresh_products(k, x - A)
-    new_max = float(np.max(P_new[:, tlo:thi + 1]))
-    old, new = region_bounds(d, k, tlo, thi, P_old, [pA])
-    assert old < new_max, (old, new_max)        # the bound without the E factors fails here
-    assert new_max <= new, (new_max, new)
-
-
-@pytest.mark.parametrize("seed", range(6))
-def test_region_bound_holds_on_random_subtractions(seed):
-    """Random pink-noise residuals, random atoms (any scale, position, bin,
-    phase and amplitude; the signal's edges included) subtracted one to three
-    at a time: every region of every scale that the reference rule marks
-    dirty keeps its new maximum below the bound."""
-    rng = np.random.default_rng(100 + seed)
-    fs, N = 64.0, 640
-    d = po.Dictionary(fs, N, 0.02)
+def _scenario(seed, fs, N, eps, rng=None):
+    """A pink-noise residual and one to three random atoms (any scale, bin
+    including 0, 1..3 and the last two, position including both edges, phase
+    and amplitude) subtracted from it: (dictionary, products before, residual
+    after, atom parameters)."""
+    rng = rng if rng is not None else np.random.default_rng(100 + seed)
+    d = po.Dictionary(fs, N, eps)
     x = synth.pink_noise(rng, N) * 2.0
     L = len(d.scales)
-    old_P = [d.fresh_products(k, x) for k in range(L)]
+    old_P = [(d.fresh_products(k, x), low_seeds(d, k, x)) for k in range(L)]
     atoms = []
     for _ in range(int(rng.integers(1, 4))):
         k = int(rng.integers(0, L))
         sc = d.scales[k]
-        f = int(rng.choice([1, 2, 3, int(rng.integers(0, sc.F)), sc.F - 2]))
+        f = int(rng.choice([0, 1, 2, 3, int(rng.integers(0, sc.F)), sc.F - 2, sc.F - 1]))
         f = max(0, min(sc.F - 1, f))
         t = int(rng.choice([0, 1, sc.T - 1, int(rng.integers(0, sc.T))]))
         w, p = _atom(d, k, f, t, float(rng.uniform(0.5, 4.0)) * np.sqrt(sc.s), float(rng.uniform(0, 2 * np.pi)))
         x = x - w
         atoms.append(p)
-    B = (N + 255) // 256
-    worst = 0.0
-    for k in range(L):
-        sc = d.scales[k]
+    return d, old_P, x, atoms
+
+
+def _regions(d, old_P, x, atoms):
+    """(scale, block, new max, old bound, new bound) of every region."""
+    out = []
+    for k in range(len(d.scales)):
         P_new = d.fresh_products(k, x)
-        for b in range(B):
+        for b in range((d.N + 255) // 256):
             tlo, thi = region_columns(d, k, b)
             if tlo > thi:
                 continue
-            t0, t1 = tlo * d.tMax / (sc.T - 1), thi * d.tMax / (sc.T - 1)
-            if not any(abs(tt - p[2] / fs) <= 3 * (sc.s + p[3] / fs) + d.tMax / (sc.T - 1)
-                       for p in atoms for tt in (t0, t1, np.clip(p[2] / fs, t0, t1))):
-                continue
-            _, new = region_bounds(d, k, tlo, thi, old_P[k], atoms)
-            new_max = float(np.max(P_new[:, tlo:thi + 1]))
-            worst = max(worst, new_max / new)
-            assert new_max <= new, (seed, k, b, new_max, new)
+            old, new = region_bounds(d, k, tlo, thi, old_P[k][0], old_P[k][1], atoms)
+            out.append((k, b, float(np.max(P_new[:, tlo:thi + 1])), old, new))
+    return out
+
+
+def test_region_bound_covers_low_bin_phase_turn():
+    """Three atoms subtracted next to the signal's start (seed 64): at bin 1 of
+    scale 18 a subtraction turns the phase and the new maximum exceeds the
+    overlap bound without the E factors by 9%; the bound with them holds for
+    every region."""
+    d, old_P, x, atoms = _scenario(64, 64.0, 640, 0.02)
+    regs = _regions(d, old_P, x, atoms)
+    missed = [(k, b, m / old) for k, b, m, old, new in regs if m > old]
+    assert any(k == 18 and r > 1.05 for k, b, r in missed), missed
+    assert all(m <= new for k, b, m, old, new in regs)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_region_bound_holds_on_random_subtractions(seed):
+    """Every region's new maximum stays below the bound, at two rates."""
+    rng = np.random.default_rng(100 + seed)
+    worst = 0.0
+    for fs, N, eps in [(64.0, 640, 0.02), (128.0, 512, 0.05)]:
+        d, old_P, x, atoms = _scenario(seed, fs, N, eps, rng)
+        for k, b, m, old, new in _regions(d, old_P, x, atoms):
+            assert m <= new, (seed, fs, k, b, m, new)
+            worst = max(worst, m / new)
     print("worst new max / bound", worst)
