@@ -36,7 +36,7 @@ EXPORTS = [
     "empi_mlp_set_params", "empi_mlp_forward", "empi_mlp_forward16", "empi_mlp_gradients", "empi_mlp_train",
     "empi_plan_tuning_size", "empi_plan_autotune", "empi_plan_get_tuning", "empi_plan_set_tuning",
     "empi_proposal_feature_dim", "empi_proposal_bound_factor", "empi_plan_set_proposal", "empi_plan_proposal_stats",
-    "empi_plan_set_training_capacity", "empi_plan_training_rows",
+    "empi_plan_set_training_capacity", "empi_plan_training_rows", "empi_decompose_noalloc",
 ]
 
 PROPOSAL_OFF, PROPOSAL_ON, PROPOSAL_COLLECT = 0, 1, 2
@@ -130,6 +130,8 @@ def lib():
         L.empi_plan_proposal_stats.argtypes = [vp, C.POINTER(C.c_ulonglong)]
         L.empi_plan_set_training_capacity.argtypes = [vp, C.c_size_t, C.c_uint]
         L.empi_plan_training_rows.argtypes = [vp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.empi_decompose_noalloc.argtypes = [C.POINTER(Item), vp, C.c_int, C.POINTER(Atom), C.POINTER(C.c_int),
+                                             dp, vp]
         L.empi_strerror.argtypes = [C.c_int]
         L.empi_strerror.restype = C.c_char_p
         L.empi_version.restype = C.c_char_p
@@ -178,6 +180,34 @@ class ItemSpec:
         return Item(self.channels, self.samples, float(self.fs), int(self.variant),
                     int(self.iteration_max), float(self.energy_percent),
                     dict_params(self.energy_error, self.scale_min, self.scale_max, self.freq_max))
+
+
+def decompose_noalloc(item, signal=None, device_ptr=None, stream=None) -> ItemResult:
+    """One item through empi_decompose_noalloc (no device allocation: the
+    library's fixed scratch array).  signal: float32 host array [C][N], or
+    device_ptr: integer device address of the same layout.  The returned
+    status is the item's; a call-level failure raises."""
+    Cn, I = item.channels, item.iteration_max
+    rows = Cn if item.variant == SMP else 1
+    atoms = (Atom * (Cn * I))()
+    counts = np.zeros(Cn, dtype=np.int32)
+    energies = np.zeros(rows * (I + 1))
+    spec = item.to_c()
+    st_ptr = C.c_void_p(stream) if stream else None
+    if device_ptr is not None:
+        ptr, on_device = C.c_void_p(int(device_ptr)), 1
+    else:
+        buf = np.ascontiguousarray(signal, dtype=np.float32).ravel()
+        if buf.size != Cn * item.samples:
+            raise ValueError(f"expected {Cn * item.samples} samples, got {buf.size}")
+        ptr, on_device = buf.ctypes.data_as(C.c_void_p), 0
+    status = lib().empi_decompose_noalloc(C.byref(spec), ptr, on_device, atoms,
+                                          counts.ctypes.data_as(C.POINTER(C.c_int)),
+                                          energies.ctypes.data_as(C.POINTER(C.c_double)), st_ptr)
+    if status not in (OK, 1, 2):
+        raise EmpiError(status, "empi_decompose_noalloc")
+    params, index = _atoms_to_arrays(atoms, (Cn, I))
+    return ItemResult(status, params, index, counts, energies.reshape(rows, I + 1))
 
 
 class Plan:

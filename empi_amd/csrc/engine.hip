@@ -21,6 +21,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <tuple>
 #include <vector>
@@ -33,6 +34,13 @@
 #include "mlp_internal.hpp"
 
 using namespace empi;
+
+// Scratch of the allocation-free entry point (empi_decompose_noalloc): every
+// device buffer of its plan is carved from this array, one copy per GPU,
+// held until the process ends.  Host code never passes the array's name to
+// the device: it uses the address hipGetSymbolAddress returns per device.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 #define EMPI_HIP(call)                                  \
     do {                                                \
@@ -1097,22 +1105,57 @@ __global__ __launch_bounds__(NT) void k_subtract_one(const UnitDev* __restrict__
 
 // ------------------------------------------------------------ host plan
 
+// Device memory carved out of one fixed region, 256-byte aligned: the
+// allocation-free entry point sets t_arena for the duration of its call, and
+// every DevBuf::alloc on that thread then takes its memory from the region
+// instead of hipMalloc.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    size_t wanted = 0;         // bytes the plan asked for, carved or not
+    void* carve(size_t bytes) {
+        const size_t off = (used + 255) & ~static_cast<size_t>(255);
+        wanted = off + bytes > wanted ? off + bytes : wanted;
+        if (off + bytes > size) {
+            used = off + bytes;
+            return nullptr;
+        }
+        used = off + bytes;
+        return base + off;
+    }
+};
+thread_local Arena* t_arena = nullptr;
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool owned = false;        // from hipMalloc (else carved from an arena)
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
     }
     empi_status alloc(size_t count) {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
+        owned = false;
         n = count;
         if (!count) return EMPI_OK;
+        if (t_arena) {
+            void* q = t_arena->carve(sizeof(T) * count);
+            if (!q) {
+                n = 0;
+                return EMPI_UNSUPPORTED;
+            }
+            p = static_cast<T*>(q);
+            return EMPI_OK;
+        }
+        owned = true;
         hipError_t e = hipMalloc(&p, sizeof(T) * count);
         if (e != hipSuccess) {
             p = nullptr;
             n = 0;
+            owned = false;
             return e == hipErrorOutOfMemory ? EMPI_NO_MEMORY : EMPI_DEVICE;
         }
         return EMPI_OK;
@@ -1242,6 +1285,8 @@ struct empi_plan {
 
     // one MP iteration (select, plan, every column class) captured once
     hipStream_t stream = nullptr;
+    // allocation-free run: the caller's stream, no graphs, no stream of its own
+    bool noalloc = false;
     hipGraphExec_t iter_exec = nullptr;
     hipGraph_t iter_graph = nullptr;
 
@@ -1268,7 +1313,7 @@ struct empi_plan {
         if (iter_graph) (void)hipGraphDestroy(iter_graph);
         if (prop_exec) (void)hipGraphExecDestroy(prop_exec);
         if (prop_graph) (void)hipGraphDestroy(prop_graph);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && !noalloc) (void)hipStreamDestroy(stream);
         for (auto& v : ev)
             for (auto& e : v) {
                 (void)hipEventDestroy(e.first);
@@ -1820,9 +1865,11 @@ empi_status plan_autotune(empi_plan* p, const float* signal, int on_device, int 
 empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream_t user) {
     const float* in = signal;
     empi_status st;
-    // work runs on the plan's stream (capturable); the caller's stream is drained first
+    // work runs on the plan's stream (capturable); the caller's stream is drained
+    // first.  An allocation-free run works on the caller's stream itself.
     EMPI_HIP(hipStreamSynchronize(user));
-    if (!p->stream) EMPI_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    if (p->noalloc) p->stream = user;
+    else if (!p->stream) EMPI_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     hipStream_t s = p->stream;
     if (!on_device) {
         if (!p->d_input.p || p->d_input.n < static_cast<size_t>(p->total_in)) {
@@ -1846,7 +1893,7 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     EMPI_HIP(hipGetLastError());
     if ((st = launch_all_columns(p, s, 1)) != EMPI_OK) return st;
     std::vector<UnitState> hs(U);
-    const bool graph = !p->profiling && std::getenv("EMPI_NO_GRAPH") == nullptr;
+    const bool graph = !p->profiling && !p->noalloc && std::getenv("EMPI_NO_GRAPH") == nullptr;
     const bool prop = p->prop_mode != EMPI_PROPOSAL_OFF;
     if (prop) {
         EMPI_HIP(hipMemsetAsync(p->d_pstats.p, 0, sizeof(unsigned long long) * 5, s));
@@ -2038,6 +2085,70 @@ empi_status empi_plan_item_result(const empi_plan* p, int i, empi_atom* atoms, i
     if (energies)
         std::memcpy(energies, p->h_energies.data() + info.energy_off, sizeof(double) * info.rows * (I + 1));
     return status;
+}
+
+namespace {
+
+// device address of `scratch` on the current device, looked up once per device
+empi_status scratch_base(void** out) {
+    constexpr int kMaxDevices = 64;
+    static void* base[kMaxDevices] = {};
+    static std::mutex lock;
+    int dev = 0;
+    EMPI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return EMPI_DEVICE;
+    std::lock_guard<std::mutex> g(lock);
+    if (!base[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            std::fprintf(stderr, "empi_hip: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+                         hipGetErrorString(e));
+            return EMPI_DEVICE;
+        }
+        base[dev] = ptr;
+    }
+    *out = base[dev];
+    return EMPI_OK;
+}
+
+struct ArenaScope {
+    explicit ArenaScope(Arena* a) { t_arena = a; }
+    ~ArenaScope() { t_arena = nullptr; }
+};
+
+empi_status too_big(const Arena& a) {
+    std::fprintf(stderr,
+                 "empi_hip: empi_decompose_noalloc: this item needs %zu bytes of device scratch, the maximum is %zu "
+                 "(unsupportedSize)\n", a.wanted, a.size);
+    return EMPI_UNSUPPORTED;
+}
+
+}  // namespace
+
+empi_status empi_decompose_noalloc(const empi_item* item, const float* signal, int signal_on_device, empi_atom* atoms,
+                                   int* counts, double* energies, void* stream) {
+    if (!item || !signal) return EMPI_INVALID_ARGUMENT;
+    void* base = nullptr;
+    empi_status st = scratch_base(&base);
+    if (st != EMPI_OK) return st;
+    Arena arena;
+    arena.base = static_cast<char*>(base);
+    arena.size = kScratchBytes;
+    ArenaScope scope(&arena);
+    try {
+        empi_plan p;
+        p.noalloc = true;
+        st = plan_build(&p, item, 1, 0);
+        if (st == EMPI_OK) st = plan_run(&p, signal, signal_on_device, static_cast<hipStream_t>(stream));
+        if (st == EMPI_UNSUPPORTED && arena.wanted > arena.size) return too_big(arena);
+        if (st != EMPI_OK) return st;
+        return empi_plan_item_result(&p, 0, atoms, counts, energies);
+    } catch (const std::bad_alloc&) {
+        return EMPI_NO_MEMORY;
+    } catch (...) {
+        return EMPI_INTERNAL;
+    }
 }
 
 size_t empi_plan_item_atom_count(const empi_plan* p, int i) {

@@ -139,6 +139,22 @@ empi_status empi_plan_run(empi_plan* plan, const float* signal, int signal_on_de
 empi_status empi_plan_item_result(const empi_plan* plan, int item, empi_atom* atoms, int* counts,
                                   double* energies);
 
+/* Allocation-free decomposition of one item (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same work and results as empi_plan_create({item}) + empi_plan_run +
+ * empi_plan_item_result + empi_plan_destroy, but every device buffer is carved
+ * from a fixed 64 MiB scratch array that the library holds on each GPU until
+ * the process ends: no hipMalloc, hipFree or stream creation, and no graph.
+ * Work runs on `stream` (a hipStream_t, NULL for the null stream); returns
+ * after it is complete.  Items whose buffers need more than 64 MiB (about
+ * 40 channels of 5120 samples; the learned proposal is not available here)
+ * return EMPI_UNSUPPORTED and print the size needed.  Calls on one GPU share
+ * the scratch and keep the last call's contents: calls that overlap on one
+ * GPU race. */
+empi_status empi_decompose_noalloc(const empi_item* item, const float* signal, int signal_on_device,
+                                   empi_atom* atoms, int* counts, double* energies, void* stream);
+
 /* Dictionary size (sum F*T) of item i; 0 on a bad index. */
 size_t      empi_plan_item_atom_count(const empi_plan* plan, int item);
 
