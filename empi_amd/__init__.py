@@ -36,7 +36,7 @@ EXPORTS = [
     "empi_mlp_set_params", "empi_mlp_forward", "empi_mlp_forward16", "empi_mlp_gradients", "empi_mlp_train",
     "empi_plan_tuning_size", "empi_plan_autotune", "empi_plan_get_tuning", "empi_plan_set_tuning",
     "empi_proposal_feature_dim", "empi_proposal_bound_factor", "empi_plan_set_proposal", "empi_plan_proposal_stats",
-    "empi_plan_set_training_capacity", "empi_plan_training_rows", "empi_decompose_noalloc",
+    "empi_plan_set_training_capacity", "empi_plan_training_rows", "empi_decompose_noalloc", "empi_plan_stage",
 ]
 
 PROPOSAL_OFF, PROPOSAL_ON, PROPOSAL_COLLECT = 0, 1, 2
@@ -96,6 +96,7 @@ def lib():
         L.empi_subtract_atom_from_signal.argtypes = [C.c_double, C.POINTER(Atom), dp, C.c_int, C.c_int]
         L.empi_plan_create.argtypes = [C.POINTER(Item), C.c_int, C.POINTER(vp)]
         L.empi_plan_run.argtypes = [vp, vp, C.c_int, vp]
+        L.empi_plan_stage.argtypes = [vp, vp]
         L.empi_plan_item_result.argtypes = [vp, C.c_int, C.POINTER(Atom), C.POINTER(C.c_int), dp]
         L.empi_plan_item_atom_count.argtypes = [vp, C.c_int]
         L.empi_plan_item_atom_count.restype = C.c_size_t

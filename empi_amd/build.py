@@ -29,7 +29,7 @@ LIB = os.path.join(HERE, "libempi_hip.so")
 HOST_SOURCES = ["host/empi_main.cpp", "host/workspace.cpp", "host/config.cpp", "host/book.cpp",
                 "host/signal_reader.cpp"]
 CLI = os.path.join(HERE, "empi-hip")
-TOOL_SOURCES = ["host/host_tool.cpp", "host/config.cpp", "host/book.cpp"]
+TOOL_SOURCES = ["host/host_tool.cpp", "host/config.cpp", "host/book.cpp", "host/signal_reader.cpp"]
 TOOL = os.path.join(HERE, "empi-host-tool")
 
 

@@ -1287,6 +1287,13 @@ struct empi_plan {
     hipStream_t stream = nullptr;
     // allocation-free run: the caller's stream, no graphs, no stream of its own
     bool noalloc = false;
+    // staged inputs (empi_plan_stage): two device buffers, one filled on the
+    // upload stream while a run reads the other
+    DevBuf<float> d_stage_buf[2];
+    int stage_ready = -1;     // slot holding a staged input not yet run
+    int stage_active = -1;    // slot the last run read
+    std::mutex stage_mu;
+    hipStream_t upload = nullptr;
     hipGraphExec_t iter_exec = nullptr;
     hipGraph_t iter_graph = nullptr;
 
@@ -1314,6 +1321,7 @@ struct empi_plan {
         if (prop_exec) (void)hipGraphExecDestroy(prop_exec);
         if (prop_graph) (void)hipGraphDestroy(prop_graph);
         if (stream && !noalloc) (void)hipStreamDestroy(stream);
+        if (upload) (void)hipStreamDestroy(upload);
         for (auto& v : ev)
             for (auto& e : v) {
                 (void)hipEventDestroy(e.first);
@@ -1871,7 +1879,14 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     if (p->noalloc) p->stream = user;
     else if (!p->stream) EMPI_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     hipStream_t s = p->stream;
-    if (!on_device) {
+    if (!signal) {
+        // the input empi_plan_stage copied (its copy is complete)
+        std::lock_guard<std::mutex> g(p->stage_mu);
+        if (p->stage_ready < 0) return EMPI_INVALID_ARGUMENT;
+        p->stage_active = p->stage_ready;
+        p->stage_ready = -1;
+        in = p->d_stage_buf[p->stage_active].p;
+    } else if (!on_device) {
         if (!p->d_input.p || p->d_input.n < static_cast<size_t>(p->total_in)) {
             if ((st = p->d_input.alloc(static_cast<size_t>(p->total_in))) != EMPI_OK) return st;
         }
@@ -2051,8 +2066,33 @@ empi_status empi_plan_create(const empi_item* items, int count, empi_plan** out)
     return EMPI_OK;
 }
 
+empi_status empi_plan_stage(empi_plan* p, const float* signal) {
+    if (!p || !signal || p->noalloc) return EMPI_INVALID_ARGUMENT;
+    int slot;
+    {
+        std::lock_guard<std::mutex> g(p->stage_mu);
+        if (p->stage_ready >= 0) return EMPI_INVALID_ARGUMENT;   // one staged input at a time
+        slot = p->stage_active == 0 ? 1 : 0;
+    }
+    try {
+        DevBuf<float>& b = p->d_stage_buf[slot];
+        empi_status st;
+        if (!b.p || b.n < static_cast<size_t>(p->total_in)) {
+            if ((st = b.alloc(static_cast<size_t>(p->total_in))) != EMPI_OK) return st;
+        }
+        if (!p->upload) EMPI_HIP(hipStreamCreateWithFlags(&p->upload, hipStreamNonBlocking));
+        if ((st = b.upload(signal, static_cast<size_t>(p->total_in), p->upload)) != EMPI_OK) return st;
+        EMPI_HIP(hipStreamSynchronize(p->upload));
+    } catch (const std::bad_alloc&) {
+        return EMPI_NO_MEMORY;
+    }
+    std::lock_guard<std::mutex> g(p->stage_mu);
+    p->stage_ready = slot;
+    return EMPI_OK;
+}
+
 empi_status empi_plan_run(empi_plan* plan, const float* signal, int signal_on_device, void* stream) {
-    if (!plan || !signal) return EMPI_INVALID_ARGUMENT;
+    if (!plan || (!signal && signal_on_device)) return EMPI_INVALID_ARGUMENT;
     try {
         return plan_run(plan, signal, signal_on_device, static_cast<hipStream_t>(stream));
     } catch (const std::bad_alloc&) {

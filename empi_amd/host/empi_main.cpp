@@ -183,9 +183,10 @@ void write_batch(const Run& r, BookWriter& writer, const std::vector<Signal>& ep
 
 // Epochs in device-resident batches of per_batch (0: all of them); output
 // replayed in reference order.  While a batch runs on the device, a reader
-// thread reads the next one.  A read error ends the stream after the epochs
-// read before it are decomposed and written, as in the reference's
-// epoch-by-epoch loop.
+// thread reads the next one and copies it to the device (empi_plan_stage)
+// when it has the current batch's shape.  A read error ends the stream after
+// the epochs read before it are decomposed and written, as in the
+// reference's epoch-by-epoch loop.
 void run_batched(Run& r, int per_batch) {
     BookWriter writer(r.bookPath);
     int samples = -1;
@@ -208,14 +209,26 @@ void run_batched(Run& r, int per_batch) {
     std::vector<Signal> epochs, next;
     std::exception_ptr error;
     read_batch(epochs, error);
-    int first = 0;   // epochs written so far
+    int first = 0;           // epochs written so far
+    bool staged = false;     // epochs' samples are on the device already
     while (!epochs.empty()) {
         std::exception_ptr next_error;
+        bool next_staged = false;
         std::thread reader;
-        if (!error) reader = std::thread([&] { read_batch(next, next_error); });
+        if (!staged) batch.prepare(epochs, r.variant, r.settings, r.dict);
+        if (!error) reader = std::thread([&] {
+            read_batch(next, next_error);
+            if (!next_error) {
+                try {
+                    next_staged = batch.stage(next, r.variant, r.settings, r.dict);
+                } catch (...) {
+                    next_error = std::current_exception();
+                }
+            }
+        });
         std::vector<BatchDecomposition::EpochResult> results;
         try {
-            results = batch.run(epochs, r.variant, r.settings, r.dict);
+            results = batch.run(epochs, r.variant, r.settings, r.dict, staged);
         } catch (...) {
             if (reader.joinable()) reader.join();
             throw;
@@ -226,6 +239,7 @@ void run_batched(Run& r, int per_batch) {
         if (error) break;
         epochs.swap(next);
         error = next_error;
+        staged = next_staged;
     }
     if (error) std::rethrow_exception(error);
     writer.close();

@@ -139,13 +139,11 @@ ChannelAtoms Mmp2Decomposition::compute(const DecompositionSettings& settings, W
 
 // ---------------------------------------------------- BatchDecomposition
 
-std::vector<BatchDecomposition::EpochResult> BatchDecomposition::run(const std::vector<Signal>& epochs, int variant,
-                                                                     const DecompositionSettings& settings,
-                                                                     const empi_dict_params& dict) {
-    std::vector<EpochResult> results(epochs.size());
-    if (epochs.empty()) return results;
+namespace {
+
+std::vector<empi_item> items_of(const std::vector<Signal>& epochs, int variant, const DecompositionSettings& settings,
+                                const empi_dict_params& dict) {
     std::vector<empi_item> items;
-    size_t total = 0;
     for (const Signal& s : epochs) {
         empi_item it{};
         it.channels = s.channels;
@@ -156,29 +154,66 @@ std::vector<BatchDecomposition::EpochResult> BatchDecomposition::run(const std::
         it.energyPercent = settings.energyPercent;
         it.dict = dict;
         items.push_back(it);
-        total += s.data.size();
     }
-    std::vector<float> input;
-    input.reserve(total);
-    for (const Signal& s : epochs)
-        for (double v : s.data) input.push_back(static_cast<float>(v));
-    auto same_item = [](const empi_item& a, const empi_item& b) {
+    return items;
+}
+
+bool same_items(const std::vector<empi_item>& x, const std::vector<empi_item>& y) {
+    auto same = [](const empi_item& a, const empi_item& b) {
         return a.channels == b.channels && a.samples == b.samples && a.fs == b.fs && a.variant == b.variant &&
                a.iterationMax == b.iterationMax && a.energyPercent == b.energyPercent &&
                a.dict.energyError == b.dict.energyError && a.dict.scaleMin == b.dict.scaleMin &&
                a.dict.scaleMax == b.dict.scaleMax && a.dict.freqMax == b.dict.freqMax;
     };
-    const bool same = items.size() == items_.size() && std::equal(items.begin(), items.end(), items_.begin(), same_item);
-    if (!same) {
-        plan_.reset();
-        items_.clear();
-        empi_plan* created = nullptr;
-        check(empi_plan_create(items.data(), static_cast<int>(items.size()), &created));
-        plan_.reset(created);
-        items_ = items;
+    return x.size() == y.size() && std::equal(x.begin(), x.end(), y.begin(), same);
+}
+
+std::vector<float> samples_of(const std::vector<Signal>& epochs) {
+    size_t total = 0;
+    for (const Signal& s : epochs) total += s.data.size();
+    std::vector<float> input;
+    input.reserve(total);
+    for (const Signal& s : epochs)
+        for (double v : s.data) input.push_back(static_cast<float>(v));
+    return input;
+}
+
+}  // namespace
+
+void BatchDecomposition::prepare(const std::vector<Signal>& epochs, int variant, const DecompositionSettings& settings,
+                                 const empi_dict_params& dict) {
+    if (epochs.empty()) return;
+    std::vector<empi_item> items = items_of(epochs, variant, settings, dict);
+    if (plan_ && same_items(items, items_)) return;
+    plan_.reset();
+    items_.clear();
+    empi_plan* created = nullptr;
+    check(empi_plan_create(items.data(), static_cast<int>(items.size()), &created));
+    plan_.reset(created);
+    items_ = items;
+}
+
+bool BatchDecomposition::stage(const std::vector<Signal>& epochs, int variant, const DecompositionSettings& settings,
+                               const empi_dict_params& dict) {
+    if (!plan_ || epochs.empty() || !same_items(items_of(epochs, variant, settings, dict), items_)) return false;
+    const std::vector<float> input = samples_of(epochs);
+    check(empi_plan_stage(plan_.get(), input.data()));
+    return true;
+}
+
+std::vector<BatchDecomposition::EpochResult> BatchDecomposition::run(const std::vector<Signal>& epochs, int variant,
+                                                                     const DecompositionSettings& settings,
+                                                                     const empi_dict_params& dict, bool staged) {
+    std::vector<EpochResult> results(epochs.size());
+    if (epochs.empty()) return results;
+    if (staged) {
+        check(empi_plan_run(plan_.get(), nullptr, 0, nullptr));
+    } else {
+        prepare(epochs, variant, settings, dict);
+        const std::vector<float> input = samples_of(epochs);
+        check(empi_plan_run(plan_.get(), input.data(), 0, nullptr));
     }
     empi_plan* plan = plan_.get();
-    check(empi_plan_run(plan, input.data(), 0, nullptr));
     const int I = settings.iterationMax;
     for (size_t e = 0; e < epochs.size(); ++e) {
         const int C = epochs[e].channels;

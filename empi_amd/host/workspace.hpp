@@ -109,8 +109,20 @@ public:
     // device buffers, captured graph) is kept and reused by the next call
     // with the same epoch count and shapes, so a stream of equal chunks
     // builds it once.
+    // staged: the epochs' samples are already on the device (stage() returned
+    // true for them, and no other run() came between).
     std::vector<EpochResult> run(const std::vector<Signal>& epochs, int variant,
-                                 const DecompositionSettings& settings, const empi_dict_params& dict);
+                                 const DecompositionSettings& settings, const empi_dict_params& dict,
+                                 bool staged = false);
+    // Builds (or keeps) the plan for these epochs; run() then reuses it.
+    void prepare(const std::vector<Signal>& epochs, int variant, const DecompositionSettings& settings,
+                 const empi_dict_params& dict);
+    // Copies the next chunk's samples to the device while run() decomposes the
+    // current one (called from the reader thread, after prepare() for the
+    // current chunk).  Returns false, staging nothing, when the chunk does not
+    // fit the current plan; run() of those epochs then stages them itself.
+    bool stage(const std::vector<Signal>& epochs, int variant, const DecompositionSettings& settings,
+               const empi_dict_params& dict);
     // ATOM lines of one progress row (src/classes.cpp:41-48), the failing
     // search's line included when status is energyMapIsEmpty.
     static void print_progress(std::ostream& out, const std::vector<double>& energies, int count, size_t atomCount,

@@ -125,9 +125,19 @@ empi_status empi_plan_create(const empi_item* items, int count, empi_plan** out)
 
 /* Decomposes the batch.  signal: float32 samples of every item, item after
  * item, each [channels][samples]; on the device when signal_on_device != 0,
- * else host memory.  stream: a hipStream_t or NULL for the null stream.
+ * else host memory; NULL (with signal_on_device 0): the input staged by
+ * empi_plan_stage.  stream: a hipStream_t or NULL for the null stream.
  * Returns after the work is complete. */
 empi_status empi_plan_run(empi_plan* plan, const float* signal, int signal_on_device, void* stream);
+
+/* Streamed input (the streamed host-to-device path of the reference's epoch
+ * reader, src/empi.cpp:145-164, one batch ahead): copies the next batch's
+ * host samples (the layout of empi_plan_run) into the plan's second input
+ * buffer on an upload stream of its own and returns when the copy is done.
+ * It may be called from another thread while empi_plan_run decomposes the
+ * previous input on the same plan; the next empi_plan_run(plan, NULL, 0, s)
+ * decomposes the staged input.  One staged input at a time. */
+empi_status empi_plan_stage(empi_plan* plan, const float* signal);
 
 /* Results of item i after empi_plan_run:
  *   atoms    [channels][iterationMax]   (row c holds counts[c] atoms)

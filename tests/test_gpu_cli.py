@@ -156,3 +156,63 @@ def test_cli_streamed_batches_match_one_batch(tmp_path, per_batch):
         assert streamed.stdout == one.stdout
         assert book.read_bytes() == one_book
         assert one.stdout.count("START\t") == 5
+
+
+def _book_epochs(path):
+    """{book epoch: [(channel, params)]} from readbook."""
+    out = subprocess.run([TOOL, "readbook", str(path)], capture_output=True, text=True, check=True).stdout
+    epochs, cur = {}, None
+    for line in out.splitlines():
+        if line.startswith("fs"):
+            continue
+        if line.startswith("epoch"):
+            cur = int(line.split()[1])
+            epochs[cur] = []
+        else:
+            r = line.split()
+            epochs[cur].append((int(r[0]), tuple(float(v) for v in r[1:])))
+    return epochs
+
+
+def test_cli_selected_epochs_and_truncated_files(tmp_path):
+    """selectedEpochs (src/empi.cpp:87-98, io.cpp:64-79) decomposes the listed
+    epochs in the listed order, each exactly as in the all-epochs run, in one
+    batch or streamed one epoch per batch with the next epoch staged on the
+    device while the current one runs; a partial last epoch, and a selected
+    epoch past the end of the file, end the run with fileIsTruncated after
+    the epochs before it (io.cpp:48-54, 64-79)."""
+    print("\n[gpu case] CLI selectedEpochs 3,1 and truncated files", flush=True)
+    N, E, C = 1024, 4, 2
+    x = np.concatenate([synth.shared_multichannel(C=C, N=N, fs=256.0, atoms=3, seed=90 + e) for e in range(E)],
+                       axis=1)                                 # [channel][E N]
+    base = (f"energyError 0.01\nmaximalNumberOfIterations 10\nenergyPercent 99\nMP SMP\nsamplingFrequency 256\n"
+            f"numberOfChannels {C}\nselectedChannels 1-{C}\nnumberOfSamplesInEpoch {N}\n")
+    all_dir = tmp_path / "all"
+    all_dir.mkdir()
+    cfg = _write_case(all_dir, x.T.copy(), base)
+    r = subprocess.run([CLI, str(cfg)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    want = _book_epochs(all_dir / "sig_smp.b")
+    assert sorted(want) == [1, 2, 3, 4]
+    for extra in ([], ["--epochs-per-batch=1"]):
+        sel = tmp_path / f"sel{len(extra)}"
+        sel.mkdir()
+        cfg = _write_case(sel, x.T.copy(), base + "selectedEpochs 3,1\n")
+        r = subprocess.run([CLI] + extra + [str(cfg)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:]
+        assert r.stdout.count("START\t") == 2
+        got = _book_epochs(sel / "sig_smp.b")
+        assert got == {1: want[3], 2: want[1]}, extra
+    # a partial last epoch: three full epochs decomposed, then the error
+    cut = tmp_path / "cut"
+    cut.mkdir()
+    cfg = _write_case(cut, x[:, :3 * N + 100].T.copy(), base)
+    for extra in ([], ["--epochs-per-batch=1"]):
+        r = subprocess.run([CLI] + extra + [str(cfg)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 1 and r.stdout.splitlines()[-1] == "ERROR\tfileIsTruncated", r.stdout[-500:]
+        assert r.stdout.count("START\t") == 3, extra
+    # a selected epoch past the end
+    cfg = _write_case(cut, x.T.copy(), base + "selectedEpochs 2,5\n")
+    r = subprocess.run([CLI, str(cfg)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and r.stdout.splitlines()[-1] == "ERROR\tfileIsTruncated", r.stdout[-500:]
+    assert r.stdout.count("START\t") == 1

@@ -100,3 +100,54 @@ def test_book_bytes_match_reference(tmp_path, fs, C, N, epochs):
                 want.append([c + 1] + [float(np.float32(v)) for v in row])
     got = [[int(r.split()[0])] + [float(np.float32(v)) for v in r.split()[1:]] for r in rows]
     assert got == want
+
+
+# ------------------------------------------------------------ signal readers
+# src/io.cpp:19-85 and src/empi.cpp:82-98: whole signal, every epoch (a
+# partial last epoch throws fileIsTruncated after the full ones), selected
+# epochs in the listed order (a listed epoch past the end throws).
+
+READ_CASES = [
+    # (mode, args, epochs expected as 0-based sample ranges, error or None)
+    ("whole", [], [(0, 23)], None),
+    ("all", ["5"], [(0, 5), (5, 10), (10, 15), (15, 20)], "fileIsTruncated"),
+    ("all", ["23"], [(0, 23)], None),
+    ("all", ["1"], [(i, i + 1) for i in range(23)], None),
+    ("selected", ["5", "3,1"], [(10, 15), (0, 5)], None),
+    ("selected", ["5", "2,2,4"], [(5, 10), (5, 10), (15, 20)], None),
+    ("selected", ["5", "4,5"], [(15, 20)], "fileIsTruncated"),
+    ("selected", ["23", "1"], [(0, 23)], None),
+]
+
+
+def _expected(x, sel, ranges, error):
+    lines = []
+    for k, (a, b) in enumerate(ranges):
+        lines.append(f"epoch {k + 1} samples {b - a}")
+        for c in sel:
+            lines.append(" ".join(f"{float(v):.9g}" for v in x[a:b, c - 1]))
+    if error:
+        lines.append(f"ERROR\t{error}")
+    return lines
+
+
+@pytest.mark.parametrize("mode,args,ranges,error", READ_CASES)
+def test_signal_readers_match_reference(tmp_path, mode, args, ranges, error):
+    rng = np.random.default_rng(len(args) * 7 + len(ranges))
+    x = rng.standard_normal((23, 4)).astype(np.float32)        # [t][channel], 4 channels
+    f = tmp_path / "sig.bin"
+    x.tofile(f)
+    sel = [4, 1, 3]
+    cmd = [str(f), "4", ",".join(map(str, sel)), mode] + args
+    mine = run([TOOL, "read"] + cmd)
+    assert mine.stdout.splitlines() == _expected(x, sel, ranges, error)
+    assert mine.returncode == (1 if error else 0)
+    driver = os.path.join(REF, "reader_driver")
+    if os.path.exists(driver):
+        ref = run([driver] + cmd)
+        assert mine.stdout == ref.stdout and mine.returncode == ref.returncode
+
+
+def test_signal_reader_missing_file(tmp_path):
+    r = run([TOOL, "read", str(tmp_path / "none.bin"), "1", "1", "whole"])
+    assert r.returncode == 1 and r.stdout.strip() == "ERROR\tcouldNotOpenSignalFile"
