@@ -368,14 +368,17 @@ __device__ inline bool sel_better(const SelBest& a, const SelBest& b) {
     return a.p > b.p || (a.p == b.p && a.key < b.key);
 }
 
-// mode bit 0: find the best atom; bit 1: subtract it and test the stop rule
+// mode bit 0: find the best atom; bit 1: subtract it and test the stop rule.
+// split: multichannel units only record the chosen entry (sel_*); k_fit_mc,
+// k_params_mc, k_sub_mc and k_stop_mc finish their iteration one workgroup
+// per channel.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units, UnitState* __restrict__ state,
                                                const ScaleDev* __restrict__ scales,
                                                const UnitScale* __restrict__ uscales,
                                                const unsigned long long* __restrict__ tilebest,
                                                double* __restrict__ residual, float* __restrict__ res32,
-                                               int mode, const RegionState* __restrict__ regions) {
+                                               int mode, const RegionState* __restrict__ regions, int split) {
     __shared__ double red[NT];
     __shared__ float redp[NT];
     __shared__ unsigned long long redk[NT];
@@ -447,6 +450,15 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
     const int k = static_cast<int>(bk >> 52);
     const int f = static_cast<int>((bk >> 26) & ((1ull << 26) - 1));
     const int t = static_cast<int>(bk & ((1ull << 26) - 1));
+    if (split && ud.C > 1) {
+        if (tid == 0) {
+            st.sel_k = k;
+            st.sel_f = f;
+            st.sel_t = t;
+            state[u] = st;
+        }
+        return;
+    }
     const ScaleDev sd = scales[ud.scale0 + k];
     double* x = residual + ud.res_off;
 
@@ -489,6 +501,96 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
         if (E / st.totalEnergy <= ud.residualEnergy) st.active = 0;
         state[u] = st;
     }
+}
+
+// ------------------------------------------------------------ multichannel units
+//
+// The rest of an iteration of the multichannel (MMP1/MMP3) units after
+// k_select's argmax, over the plan's list of such units: the FP64 refit and
+// the subtraction take one workgroup per (channel, unit) instead of one per
+// unit; the channel energies are summed in the same order as unit_energy.
+
+// FP64 map value of the chosen entry, channel blockIdx.x of unit mc[blockIdx.y]
+template <int NT>
+__global__ __launch_bounds__(NT) void k_fit_mc(const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
+                                               const ScaleDev* __restrict__ scales, const int* __restrict__ mc,
+                                               const double* __restrict__ residual) {
+    __shared__ double red[NT];
+    const int u = mc[blockIdx.y], c = blockIdx.x;
+    const UnitDev ud = units[u];
+    const UnitState st = state[u];
+    if (c >= ud.C || !st.active) return;
+    const ScaleDev sd = scales[ud.scale0 + st.sel_k];
+    UnitDev one = ud;
+    one.C = 1;
+    fit_column<NT>(one, sd, residual + ud.res_off + static_cast<long long>(c) * ud.N, st.sel_f, st.sel_t, red,
+                   ud.fitv + c);
+}
+
+// atom parameters of every channel (MMP1 constraint included); the atom of
+// the last iteration is not subtracted (classes.cpp:57-59)
+__global__ void k_params_mc(const UnitDev* __restrict__ units, UnitState* __restrict__ state,
+                            const ScaleDev* __restrict__ scales, const int* __restrict__ mc, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int u = mc[i];
+    const UnitDev ud = units[u];
+    UnitState st = state[u];
+    if (!st.active) return;
+    const ScaleDev sd = scales[ud.scale0 + st.sel_k];
+    atom_params(ud, sd, st.sel_k, st.sel_f, st.sel_t, ud.fitv, ud.atoms, st.iteration);
+    if (st.iteration + 1 >= ud.iterMax) {
+        st.iteration += 1;
+        st.active = 0;
+        state[u] = st;
+    }
+}
+
+// residual update of channel blockIdx.x and its energy sum_n x^2
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sub_mc(const UnitDev* __restrict__ units, const UnitState* __restrict__ state,
+                                               const int* __restrict__ mc, double* __restrict__ residual,
+                                               float* __restrict__ res32, double* __restrict__ epart, int cmax) {
+    __shared__ double red[NT];
+    __shared__ empi_atom a;
+    const int u = mc[blockIdx.y], c = blockIdx.x;
+    const UnitDev ud = units[u];
+    const UnitState st = state[u];
+    if (c >= ud.C || !st.active) return;
+    if (threadIdx.x == 0) a = ud.atoms[static_cast<long long>(c) * ud.atom_cstride + st.iteration];
+    __syncthreads();
+    const long long off = ud.res_off + static_cast<long long>(c) * ud.N;
+    subtract_waveform<NT>(a, ud.fs, residual + off, ud.N, res32 + off, st.xscale);
+    __syncthreads();
+    double part = 0.0;
+    for (int n = threadIdx.x; n < ud.N; n += NT) {
+        const double v = residual[off + n];
+        part += v * v;
+    }
+    const double e = block_sum<NT>(part, red);
+    if (threadIdx.x == 0) epart[static_cast<long long>(blockIdx.y) * cmax + c] = e;
+}
+
+// residual energy and the stop test (classes.cpp:61-70)
+__global__ void k_stop_mc(const UnitDev* __restrict__ units, UnitState* __restrict__ state, const int* __restrict__ mc,
+                          int count, const double* __restrict__ epart, int cmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int u = mc[i];
+    const UnitDev ud = units[u];
+    UnitState st = state[u];
+    if (!st.active) return;
+    double E = 0.0;
+    for (int c = 0; c < ud.C; ++c) E += epart[static_cast<long long>(i) * cmax + c] / ud.fs;
+    const int it = st.iteration;
+    const empi_atom& a = ud.atoms[it];
+    st.iteration = it + 1;
+    st.residEnergy = E;
+    st.tA = a.params[2] / ud.fs;
+    st.sA = a.params[3] / ud.fs;
+    ud.energies[it + 1] = E;
+    if (E / st.totalEnergy <= ud.residualEnergy) st.active = 0;
+    state[u] = st;
 }
 
 // ------------------------------------------------------------ proposal
@@ -1244,6 +1346,11 @@ struct empi_plan {
     DevBuf<double> d_scratch;
     DevBuf<float> d_input;
     DevBuf<unsigned long long> d_counter;
+    // multichannel units, whose selection step runs one workgroup per channel
+    std::vector<int> mc_units;
+    int mc_cmax = 0;
+    DevBuf<int> d_mc;
+    DevBuf<double> d_epart;               // [multichannel unit][channel] energy sums
 
     // proposal mode (single-channel units)
     std::vector<RegionDev> regtab_h;
@@ -1595,6 +1702,14 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_scratch.alloc(static_cast<size_t>(scratch_off))) != EMPI_OK) return st;
     if ((st = p->d_counter.alloc(2 * kNumClasses)) != EMPI_OK) return st;
     for (size_t ui = 0; ui < p->units.size(); ++ui) {
+        if (p->units[ui].C > 1) {
+            p->mc_units.push_back(static_cast<int>(ui));
+            p->mc_cmax = std::max(p->mc_cmax, p->units[ui].C);
+        }
+    }
+    if ((st = p->d_mc.alloc(p->mc_units.size())) != EMPI_OK) return st;
+    if ((st = p->d_epart.alloc(p->mc_units.size() * static_cast<size_t>(p->mc_cmax))) != EMPI_OK) return st;
+    for (size_t ui = 0; ui < p->units.size(); ++ui) {
         UnitDev& u = p->units[ui];
         u.atoms = p->d_atoms.p + reinterpret_cast<long long>(u.atoms);
         u.energies = p->d_energies.p + reinterpret_cast<long long>(u.energies);
@@ -1604,6 +1719,7 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_scales.upload(p->scales_h.data(), p->scales_h.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_units.upload(p->units.data(), p->units.size(), s0)) != EMPI_OK) return st;
     if ((st = p->d_etab.upload(etab_h.data(), etab_h.size(), s0)) != EMPI_OK) return st;
+    if ((st = p->d_mc.upload(p->mc_units.data(), p->mc_units.size(), s0)) != EMPI_OK) return st;
     std::vector<ClassDev> cds;
     for (int mode = 0; mode < 3; ++mode) {
         for (int c = 0; c < kNumClasses; ++c) {
@@ -1666,6 +1782,28 @@ empi_status launch_class_columns(empi_plan* p, hipStream_t s) {
                                               p->d_res32.p, p->d_cells.p, p->d_tiles.p,
                                               p->d_counter.p, low ? p->d_celllow.p : nullptr);
         if (evp) EMPI_HIP(hipEventRecord(evp->second, s));
+    }
+    EMPI_HIP(hipGetLastError());
+    return EMPI_OK;
+}
+
+// The selection step of one iteration: k_select for every unit; the
+// multichannel units then finish it one workgroup per channel.
+empi_status launch_select(empi_plan* p, hipStream_t s) {
+    const int U = static_cast<int>(p->units.size());
+    const int M = static_cast<int>(p->mc_units.size());
+    hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
+                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3, nullptr, M > 0 ? 1 : 0);
+    if (M > 0) {
+        const dim3 grid(p->mc_cmax, M);
+        hipLaunchKernelGGL(k_fit_mc<kSelectThreads>, grid, dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
+                           p->d_scales.p, p->d_mc.p, p->d_res.p);
+        hipLaunchKernelGGL(k_params_mc, dim3((M + 63) / 64), dim3(64), 0, s, p->d_units.p, p->d_state.p,
+                           p->d_scales.p, p->d_mc.p, M);
+        hipLaunchKernelGGL(k_sub_mc<kSelectThreads>, grid, dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
+                           p->d_mc.p, p->d_res.p, p->d_res32.p, p->d_epart.p, p->mc_cmax);
+        hipLaunchKernelGGL(k_stop_mc, dim3((M + 63) / 64), dim3(64), 0, s, p->d_units.p, p->d_state.p, p->d_mc.p, M,
+                           p->d_epart.p, p->mc_cmax);
     }
     EMPI_HIP(hipGetLastError());
     return EMPI_OK;
@@ -1765,7 +1903,7 @@ empi_status launch_proposal_iteration(empi_plan* p, hipStream_t s) {
     const int U = static_cast<int>(p->units.size());
     empi_status st;
     hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
-                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3, p->d_regions.p);
+                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3, p->d_regions.p, 0);
     hipLaunchKernelGGL(k_mark_dirty, dim3(U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p, p->d_scales.p,
                        p->d_regtab.p, p->d_regions.p);
     EMPI_HIP(hipGetLastError());
@@ -1930,9 +2068,8 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     if (graph && !prop && !p->iter_exec) {
         // capture one iteration; every kernel reads its per-iteration state from HBM
         EMPI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
-                           p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3, nullptr);
-        empi_status cst = launch_all_columns(p, s, 0);
+        empi_status cst = launch_select(p, s);
+        if (cst == EMPI_OK) cst = launch_all_columns(p, s, 0);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &g);
         if (cst != EMPI_OK) return cst;
@@ -1950,9 +2087,7 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
         } else if (graph) {
             EMPI_HIP(hipGraphLaunch(p->iter_exec, s));
         } else {
-            hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(U), dim3(kSelectThreads), 0, s, p->d_units.p,
-                               p->d_state.p, p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 3, nullptr);
-            EMPI_HIP(hipGetLastError());
+            if ((st = launch_select(p, s)) != EMPI_OK) return st;
             if (it + 1 < p->maxIter) {
                 if ((st = launch_all_columns(p, s, 0)) != EMPI_OK) return st;
             }
@@ -2408,7 +2543,7 @@ empi_status empi_ws_find_best(empi_ws* w, empi_atom* out) {
     us.status = EMPI_OK;
     EMPI_HIP(hipMemcpyAsync(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_select<kSelectThreads>, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p,
-                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 1, nullptr);
+                       p->d_scales.p, p->d_uscales.p, p->d_tiles.p, p->d_res.p, p->d_res32.p, 1, nullptr, 0);
     EMPI_HIP(hipGetLastError());
     EMPI_HIP(hipMemcpyAsync(&us, p->d_state.p, sizeof us, hipMemcpyDeviceToHost, s));
     const int I = p->items[0].item.iterationMax;

@@ -75,6 +75,8 @@ struct UnitState {
     double tA, sA;            // last subtracted atom, seconds
     float xscale;             // power-of-two input scaling of the FP32 maps
     int pad2;
+    int sel_k, sel_f, sel_t;  // the chosen entry, handed from k_select to the per-channel kernels
+    int sel_pad;
 };
 
 struct Job {                  // one (unit, scale) pair in a kernel class
