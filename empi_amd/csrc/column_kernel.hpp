@@ -582,11 +582,19 @@ void launch_columns(int grid, hipStream_t s, const ClassDev* classes, int cls, c
                        state, residual, cellbest, tilebest, counter, celllow);
 }
 
-// the launcher of class Q in a multichannel mode
+// the launcher of class Q in a multichannel mode; variant v allows
+// column_waves + v waves per SIMD
+template <int Q, int MODE>
+ColumnLauncher multichannel_variant(int variant) {
+    constexpr int W = column_waves<Q, MODE>();
+    if (variant == 1) return launch_columns<Q, MODE, W + 1, false>;
+    if (variant == 2) return launch_columns<Q, MODE, W + 2, false>;
+    return launch_columns<Q, MODE, W, false>;
+}
 template <int Q>
-ColumnLauncher multichannel_launcher_for(int mode) {
-    if (mode == kModeSum) return launch_columns<Q, kModeSum, column_waves<Q, kModeSum>(), false>;
-    if (mode == kModeMmp1) return launch_columns<Q, kModeMmp1, column_waves<Q, kModeMmp1>(), false>;
+ColumnLauncher multichannel_launcher_for(int mode, int variant) {
+    if (mode == kModeSum) return multichannel_variant<Q, kModeSum>(variant);
+    if (mode == kModeMmp1) return multichannel_variant<Q, kModeMmp1>(variant);
     return nullptr;
 }
 
@@ -594,7 +602,7 @@ ColumnLauncher multichannel_launcher_for(int mode) {
 // kernel that also writes the cells' low-bin bests
 template <int Q>
 ColumnLauncher launcher_for(int mode, int variant, bool low) {
-    if (mode != kModeSingle) return low ? nullptr : multichannel_launcher_for<Q>(mode);
+    if (mode != kModeSingle) return low ? nullptr : multichannel_launcher_for<Q>(mode, variant);
     if (low) {
         if (variant == 1) return launch_columns<Q, kModeSingle, variant_waves<Q, 1>(), true>;
         if (variant == 2) return launch_columns<Q, kModeSingle, variant_waves<Q, 2>(), true>;

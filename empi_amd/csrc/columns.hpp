@@ -39,11 +39,13 @@ struct Geo {
 // multichannel workspaces); MMP1 constraint (three accumulators per bin)
 enum { kModeSingle = 0, kModeSum = 1, kModeMmp1 = 2 };
 // waves per SIMD the register allocator must allow in the multichannel modes
+// (the default; tuning variants add one or two)
 template <int Q, int MODE>
 constexpr int column_waves() { return MODE == kModeSum ? 2 : 1; }
 
-// Tuning variants of the single-channel column kernels: waves per SIMD the
-// register allocator must allow (4, 3, 2), times two persistent grid sizes.
+// Tuning variants of the column kernels: waves per SIMD the register
+// allocator must allow (single channel 4, 3, 2; multichannel the mode's
+// column_waves + 0, 1, 2), times two persistent grid sizes.
 constexpr int kWaveVariants = 3;
 constexpr int kGridVariants = 2;
 constexpr int kTuneChoices = kWaveVariants * kGridVariants;

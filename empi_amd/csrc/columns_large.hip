@@ -8,7 +8,7 @@ ColumnLauncher column_launcher_large(int lgQ, int mode, int variant, bool low) {
     switch (lgQ) {
     case 11: return launcher_for<2048>(mode, variant, low);
     case 12: return launcher_for<4096>(mode, variant, low);
-    case 13: return (low ? nullptr : multichannel_launcher_for<8192>(mode));
+    case 13: return (low ? nullptr : multichannel_launcher_for<8192>(mode, variant));
     default: return nullptr;
     }
 }

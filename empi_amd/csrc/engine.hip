@@ -1974,32 +1974,55 @@ empi_status plan_autotune(empi_plan* p, const float* signal, int on_device, int 
     EMPI_HIP(hipEventCreate(&e0));
     EMPI_HIP(hipEventCreate(&e1));
     empi_status result = EMPI_OK;
-    for (int mode = 0; mode < 3 && result == EMPI_OK; ++mode) {
-        for (int c = 0; c < kNumClasses && result == EMPI_OK; ++c) {
-            ClassJobs& cj = p->classes[mode][c];
-            if (cj.jobs.empty() || cj.tiles == 0) continue;
-            const int n = mode == kModeSingle ? kTuneChoices : kGridVariants;
-            float best = INFINITY;
-            int best_choice = p->tune[mode][c];
-            for (int v = 0; v < n && result == EMPI_OK; ++v) {
-                const int choice = mode == kModeSingle ? v : v * kWaveVariants;
-                for (int r = 0; r < std::max(1, reps); ++r) {
-                    if (hipEventRecord(e0, s) != hipSuccess) result = EMPI_DEVICE;
-                    column_launcher(mode, c, choice)(column_grid(1 << (c + kMinLgQ), cj.tiles, choice / kWaveVariants),
-                                                     s, p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p,
-                                                     p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p,
-                                                     nullptr, nullptr);
-                    float ms = 0.0f;
-                    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-                        result = EMPI_DEVICE;
-                    if (result == EMPI_OK && ms < best) {
-                        best = ms;
-                        best_choice = choice;
+    // every variant of every class on the current work lists, best of reps
+    static thread_local float ms[2][3][kNumClasses][kTuneChoices];
+    auto time_lists = [&](int pass) {
+        for (int mode = 0; mode < 3 && result == EMPI_OK; ++mode) {
+            for (int c = 0; c < kNumClasses && result == EMPI_OK; ++c) {
+                ClassJobs& cj = p->classes[mode][c];
+                if (cj.jobs.empty() || cj.tiles == 0) continue;
+                for (int v = 0; v < kTuneChoices && result == EMPI_OK; ++v) {
+                    float best = INFINITY;
+                    for (int r = 0; r < std::max(1, reps); ++r) {
+                        if (hipEventRecord(e0, s) != hipSuccess) result = EMPI_DEVICE;
+                        column_launcher(mode, c, v)(column_grid(1 << (c + kMinLgQ), cj.tiles, v / kWaveVariants), s,
+                                                    p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p,
+                                                    p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p, nullptr,
+                                                    nullptr);
+                        float t = 0.0f;
+                        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+                            result = EMPI_DEVICE;
+                        best = std::min(best, t);
                     }
+                    ms[pass][mode][c][v] = best;
                 }
             }
-            p->tune[mode][c] = best_choice;
+        }
+    };
+    // the full map build, then the strips one iteration re-evaluates (the
+    // iteration's selection and work lists from a full build with the
+    // current table); a class re-run on the same list recomputes the same cells
+    time_lists(0);
+    if (result == EMPI_OK) result = launch_class_columns(p, s);
+    if (result == EMPI_OK) result = launch_select(p, s);
+    if (result == EMPI_OK && p->nclass_dev)
+        hipLaunchKernelGGL(k_plan, dim3(p->nclass_dev), dim3(kPlanThreads), 0, s, p->d_classes.p, p->d_state.p,
+                           p->d_scales.p, 0);
+    if (result == EMPI_OK) time_lists(1);
+    // keep the variant with the least full build + iterationMax strips
+    for (int mode = 0; mode < 3 && result == EMPI_OK; ++mode) {
+        for (int c = 0; c < kNumClasses; ++c) {
+            ClassJobs& cj = p->classes[mode][c];
+            if (cj.jobs.empty() || cj.tiles == 0) continue;
+            double best = INFINITY;
+            for (int v = 0; v < kTuneChoices; ++v) {
+                const double cost = ms[0][mode][c][v] + static_cast<double>(p->maxIter) * ms[1][mode][c][v];
+                if (cost < best) {
+                    best = cost;
+                    p->tune[mode][c] = v;
+                }
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -2387,7 +2410,7 @@ empi_status empi_plan_set_tuning(empi_plan* p, const int* in) {
     for (int m = 0; m < 3; ++m)
         for (int c = 0; c < kNumClasses; ++c) {
             const int v = in[m * kNumClasses + c];
-            if (v < 0 || v >= kTuneChoices || (m != kModeSingle && v % kWaveVariants != 0)) return EMPI_INVALID_ARGUMENT;
+            if (v < 0 || v >= kTuneChoices) return EMPI_INVALID_ARGUMENT;
         }
     for (int m = 0; m < 3; ++m)
         for (int c = 0; c < kNumClasses; ++c) p->tune[m][c] = in[m * kNumClasses + c];

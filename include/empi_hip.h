@@ -185,8 +185,9 @@ empi_status empi_plan_kernel_stats(const empi_plan* plan, int cls, int* sub_leng
 void        empi_plan_destroy(empi_plan* plan);
 
 /* Autotuner (no counterpart in the reference).  Each column-kernel class has
- * variants: waves per SIMD the register allocator must allow (4, 3 or 2) for
- * single-channel units, and two persistent grid sizes.  empi_plan_autotune
+ * variants: waves per SIMD the register allocator must allow (4, 3 or 2 for
+ * single-channel units, the mode's default plus 0, 1 or 2 for multichannel
+ * ones), and two persistent grid sizes.  empi_plan_autotune
  * times every variant of every class on the full map build of this batch
  * (signal as for empi_plan_run; best of `repeats`, HIP events on the plan's
  * stream) and keeps the fastest; the table, empi_plan_tuning_size() ints

@@ -42,6 +42,36 @@ def test_autotune_keeps_results():
         for a, b in zip(i0, i2):
             assert np.array_equal(a, b), v
     bad = table.copy()
-    bad[classes] = 1     # multi-channel classes take grid variants only
+    bad[classes] = 6     # six variants per class
     with pytest.raises(E.EmpiError):
         plan.set_tuning(bad)
+
+
+def test_multichannel_variants_keep_results():
+    """Every wave and grid variant of the sum (MMP3) and MMP1 classes gives the
+    same atoms."""
+    _announce("autotune: MMP3 and MMP1 items, 4 ch x 2048, 12 iterations, every multichannel variant")
+    specs = [E.ItemSpec(4, 2048, 256.0, E.MMP3, 12, 99.9, 0.01), E.ItemSpec(4, 2048, 256.0, E.MMP1, 12, 99.9, 0.01)]
+    xs = [synth.shared_multichannel(C=4, N=2048, fs=256.0, atoms=4, noise_sigma=0.1, seed=40 + i) for i in range(2)]
+    flat = np.concatenate([x.ravel() for x in xs]).astype(np.float32)
+    plan = E.Plan(specs)
+    plan.run(flat)
+    p0, i0 = _atoms(plan, 2)
+    classes = E.lib().empi_plan_kernel_classes()
+    table = plan.tuning()
+    for v in range(6):
+        t = table.copy()
+        t[classes:] = v
+        plan.set_tuning(t)
+        plan.run(flat)
+        p2, i2 = _atoms(plan, 2)
+        for a, b in zip(i0, i2):
+            assert np.array_equal(a, b), v
+        for a, b in zip(p0, p2):
+            assert np.array_equal(a, b), v
+    tuned = plan.autotune(flat, repeats=1)
+    print("tuning", tuned.tolist(), flush=True)
+    plan.run(flat)
+    p3, _ = _atoms(plan, 2)
+    for a, b in zip(p0, p3):
+        assert np.array_equal(a, b)
