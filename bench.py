@@ -256,7 +256,7 @@ def headline(args, torch):
     return line, cases, float(np.mean([int(r.counts.sum()) for r in res]))
 
 
-def smp_batch(args, torch, steps):
+def smp_batch(args, torch, steps, warmup=1, tuning=None):
     import empi_amd as E
     from empi_amd import synth
     fs, N, C, items = 256.0, 5120, args.channels, args.epochs
@@ -269,8 +269,12 @@ def smp_batch(args, torch, steps):
     dev = torch.from_numpy(np.ascontiguousarray(sig, np.float32).ravel()).to("cuda:0")
     torch.cuda.synchronize()
     ptr = dev.data_ptr()
-    tuning = plan.autotune(device_ptr=ptr, repeats=2)
-    total, per = run_steps(plan, ptr, torch, 1, steps, "smp-batch")
+    if tuning:
+        plan.set_tuning([int(v) for v in tuning.split(",")])
+        tuning = plan.tuning()
+    else:
+        tuning = plan.autotune(device_ptr=ptr, repeats=2)
+    total, per = run_steps(plan, ptr, torch, warmup, steps, "smp-batch")
     atoms, expl, _ = item_stats(plan, len(specs))
     wid = SMP_ID if (C, items, I) == (64, 20, 100) else None
     roof, kernels = roofline(plan, ptr, wid)
@@ -287,6 +291,7 @@ def smp_batch(args, torch, steps):
 
 def single_plan(args, torch):
     """--workload single | mmp3 | uneven: one plan, plain method."""
+    assert args.workload in ("single", "mmp3", "uneven"), args.workload
     import empi_amd as E
     from empi_amd import synth
     if args.workload == "single":
@@ -350,6 +355,17 @@ def main():
     base = {"metric": METRIC, "unit": "atoms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "fp32 maps / fp64 energy", "data": "synthetic"}
+    if args.workload == "smp-batch":
+        sec, _ = smp_batch(args, torch, args.steps, args.warmup, args.tuning)
+        line = {"value": sec["atoms_per_s"], "ms_per_step": float(np.median(sec["step_seconds"])) * 1e3,
+                "step_seconds": sec["step_seconds"],
+                "config": {"workload": sec["workload"], "workload_id": sec["workload_id"], "method": "plain",
+                           "items": args.epochs, "atoms_per_step": sec["atoms_per_step"],
+                           "mean_explained_energy": sec["mean_explained_energy"],
+                           "autotuned_variants": sec["autotuned_variants"]},
+                "roofline": sec["roofline"], "kernels": sec["kernels"]}
+        print(json.dumps({**base, **line}))
+        return
     if args.workload != "proposal":
         line = single_plan(args, torch)
         print(json.dumps({**base, **line}))

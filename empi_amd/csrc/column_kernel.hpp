@@ -128,6 +128,14 @@ struct CellInfo {
     int ka, kb;     // sub-transforms of the cell's two slots, -1 = none
 };   // 28 bytes: the Q = 16 and 32 classes fit four workgroups per CU
 
+// bins a re-evaluated cell contributes per pass: k = P k1 + k2 (k1 < Q) below
+// F for each of its sub-transforms k2, and the Nyquist bin k = M of a g = 0
+// cell when F > M
+__device__ inline unsigned cell_bins(const CellInfo& ci, int P, int Q, int F, int M) {
+    auto below = [&](int k2) { return k2 < 0 || k2 >= F ? 0 : min(Q, (F - k2 + P - 1) / P); };
+    return static_cast<unsigned>(below(ci.ka) + below(ci.kb) + (ci.ka == 0 && F > M ? 1 : 0));
+}
+
 // MODE kModeSingle: one channel, no constraint (SMP units, MMP2 sums);
 // kModeSum: sum over channels of the products (MMP3); kModeMmp1: MMP1
 // constraint on low-frequency bins, channel sum on high ones.
@@ -195,6 +203,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         const double s = sd.s, fs = sd.fs;
         const int P = sd.P, G = sd.G, M = sd.M, F = sd.F, Nfft = sd.Nfft;
 
+        // bins this thread's cells contribute to the evaluation count
+        unsigned mycnt = 0;
         for (int c = tid; c < NCELL; c += NT) {
             CellInfo ci{};
             const int cell = cell0 + c;
@@ -227,6 +237,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             info[c] = ci;
             best[c] = 0ull;
             if constexpr (LB) lowbest[c] = 0u;
+            if (ci.valid) mycnt += cell_bins(ci, P, Q, F, M);
         }
         if (tid == 0) {
             cnt_lds = 0;
@@ -239,6 +250,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
 
         const float invN = ldexpf(1.0f, -sd.lgNfft);   // Nfft is a power of two
         const float invM = 2.0f * invN;
+        const float Kq = 0.25f * sd.Kp;
         constexpr bool mmp1 = MODE == kModeMmp1;
         const int C = MULTI ? ud.C : 1;
 
@@ -248,42 +260,55 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         for (int i = 0; i < NACC; ++i) acc0[i] = 0.0f;
 #pragma unroll
         for (int i = 0; i < NACC3; ++i) acc1[i] = acc2[i] = 0.0f;
-        unsigned mycnt = 0;
 
         for (int ch = 0; ch < C; ++ch) {
-            const float* x = residual + ud.res_off + static_cast<long long>(ch) * ud.N;
             // load: packed pairs z[m] = w[2m] + i w[2m+1], pre-twiddled per slot.
-            // Windowed pair mm of a cell; samples beyond the window are zero.
-            // Unconditional loads at a clamped index: no branch around them,
-            // so the unrolled iterations' loads overlap.
-            auto window_pair = [&](const CellInfo& ci, const int mm) {
-                const int n0 = 2 * mm, n1 = 2 * mm + 1;
-                const bool v0 = ci.valid && n0 < ci.W, v1 = ci.valid && n1 < ci.W;
-                const float x0 = x[v0 ? ci.iL + n0 : 0];
-                const float x1 = x[v1 ? ci.iL + n1 : 0];
-                float w0 = 0.f, w1 = 0.f;
-                if (v0) {
-                    const float u = static_cast<float>(n0 - ci.dint) - ci.dfrac;
-                    w0 = x0 * __expf(-sd.gA * u * u);
-                }
-                if (v1) {
-                    const float u = static_cast<float>(n1 - ci.dint) - ci.dfrac;
-                    w1 = x1 * __expf(-sd.gA * u * u);
-                }
-                return make_float2(w0, w1);
+            // The channel is read as a buffer resource: a sample at or past a
+            // cell's window length (every sample of a cell not re-evaluated)
+            // is read at an offset past the channel, which returns zero, so
+            // the loads need no branch and the unrolled iterations overlap.
+            const __amdgpu_buffer_rsrc_t xr =
+                buffer_of(residual + ud.res_off + static_cast<long long>(ch) * ud.N, ud.N * 4);
+            const unsigned oob = static_cast<unsigned>(ud.N) * 4u;
+            const float sg = sd.sgA;
+            // window samples 2mm and 2mm + 1 of a cell: x exp2(-(sgA u)^2),
+            // u = n - dint - dfrac, with sgA (n - dint) - sgA dfrac in one fma
+            auto window_pair = [&](const CellInfo& ci, const int lim, const float cg, const int mm) {
+                const int n0 = 2 * mm;
+                const unsigned o = static_cast<unsigned>(ci.iL + n0) * 4u;
+                const float x0 = buffer_load(xr, n0 < lim ? o : oob);
+                const float x1 = buffer_load(xr, n0 + 1 < lim ? o + 4u : oob);
+                const float s0 = fmaf(sg, static_cast<float>(n0 - ci.dint), cg);
+                const float s1 = s0 + sg;
+                return make_float2(x0 * __builtin_amdgcn_exp2f(-(s0 * s0)), x1 * __builtin_amdgcn_exp2f(-(s1 * s1)));
             };
             constexpr int GL = Q >= NT ? Q / NT : 1;   // consecutive pairs sharing a cell
 #pragma unroll
             for (int gi = 0; gi < EPT / 2 / GL; ++gi) {
                 const int c = (tid + gi * GL * NT) / Q;
                 const CellInfo ci = info[c];
+                const int lim = ci.valid ? ci.W : 0;
+                const float cg = -sg * ci.dfrac;
+                const int m0 = (tid + gi * GL * NT) % Q;
+                // pre-twiddles w_M^{k2 m} from exact revolution counts (k2 m < M <
+                // 2^24) at the group's first pair; the GL pairs of a group are NT
+                // apart in one cell, so later ones advance by w_M^{k2 NT}
+                const unsigned um0 = static_cast<unsigned>(m0);
+                const unsigned uka = static_cast<unsigned>(ci.ka), ukb = static_cast<unsigned>(max(ci.kb, 0));
+                float2 twa = cis_rev(-static_cast<float>(__umul24(uka, um0)) * invM);
+                float2 twb = cis_rev(-static_cast<float>(__umul24(ukb, um0)) * invM);
+                float2 sta = make_float2(1.f, 0.f), stb = make_float2(1.f, 0.f);
+                if constexpr (GL > 1) {
+                    sta = cis_rev(-static_cast<float>(__umul24(uka, static_cast<unsigned>(NT))) * invM);
+                    stb = cis_rev(-static_cast<float>(__umul24(ukb, static_cast<unsigned>(NT))) * invM);
+                }
 #pragma unroll 2
                 for (int q = 0; q < GL; ++q) {
-                    const int m = (tid + (gi * GL + q) * NT) % Q;
-                    float2 z = window_pair(ci, m);
+                    const int m = m0 + q * NT;   // GL > 1: one cell, no wrap
                     float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
-                    // pairs past the window (and every later section) are zero: no twiddles
-                    if (ci.valid && 2 * m < ci.W) {
+                    // pairs past the window (and every later section) are zero
+                    if (2 * m < lim) {
+                        float2 z = window_pair(ci, lim, cg, m);
                         float2 zin_b = z;
                         if constexpr (Q == kMaxSubTransform) {
                             // input m + f Q enters sub-transform k2 with e^{-2 pi i k2 f / P}
@@ -295,7 +320,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                                         wa = cmul(wa, ta);
                                         wb = cmul(wb, tb);
                                     }
-                                    const float2 zf = window_pair(ci, m + f * Q);
+                                    const float2 zf = window_pair(ci, lim, cg, m + f * Q);
                                     sa = cadd(sa, cmul(zf, wa));
                                     sb = cadd(sb, cmul(zf, wb));
                                 }
@@ -303,15 +328,19 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                                 zin_b = sb;
                             }
                         }
-                        // pre-twiddle w_M^{k2 m}, k2 m < M: exact revolution counts
-                        const unsigned um = static_cast<unsigned>(m);   // k2 m < M < 2^24
-                        za = ci.ka == 0 ? z
-                                        : cmul(z, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.ka), um)) * invM));
-                        if (ci.kb >= 0)
-                            zb = cmul(zin_b, cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(ci.kb), um)) * invM));
+                        za = ci.ka == 0 ? z : cmul(z, twa);
+                        if (ci.kb >= 0) zb = cmul(zin_b, twb);
                     }
-                    buf[lpad<Q>((2 * c) * Q + m)] = za;
-                    buf[lpad<Q>((2 * c + 1) * Q + m)] = zb;
+                    const int pa = lpad<Q>((2 * c) * Q + m);
+                    buf[pa] = za;
+                    if constexpr (Q % (1 << pad_shift<Q>()) == 0)
+                        buf[pa + Q + (Q >> pad_shift<Q>())] = zb;   // lpad(i + Q) = lpad(i) + lpad(Q)
+                    else
+                        buf[lpad<Q>((2 * c + 1) * Q + m)] = zb;
+                    if constexpr (GL > 1) {
+                        if (ci.ka != 0) twa = cmul(twa, sta);
+                        twb = cmul(twb, stb);
+                    }
                 }
             }
             __syncthreads();
@@ -352,7 +381,6 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                         if (j >= 1 && j < sd.JB) low = fmaxf(low, num * sd.Kp);
                 }
                 if constexpr (!MULTI) {
-                    ++mycnt;
                     run = max(run, pack_best(pe, static_cast<unsigned>(k)));
                 } else if (mmp1 && !high) {
                     if constexpr (mmp1) {
@@ -382,14 +410,17 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const float Xr = 0.5f * (Ar + pi), Xi = 0.5f * (Ai - pr);
                 from_x(k, Xr, Xi, ci, i, run, low);
             };
-            // Two bins from one pair of LDS values (single-channel units,
-            // Q >= NT): bin k and its mirror M - k share A = Z[k] + conj Z[M-k],
+            // Two bins from one pair of LDS values (single-channel units):
+            // bin k and its mirror M - k share A = Z[k] + conj Z[M-k],
             // B = Z[k] - conj Z[M-k] and w^k: X[k] = (A - i w^k B) / 2,
             // X[M-k] = conj(A + i w^k B) / 2.  Pair u < Q of cell c is (slot a, u)
             // with (slot b, Q-1-u) for g > 0; for g = 0, (a, u) with (a, Q-u) for
             // u < Q/2 and (b, u') with (b, Q-1-u'), u' = u - Q/2; u = 0 also takes
             // the self-mirrored (a, Q/2).  The Nyquist bin keeps its own path.
-            auto pair = [&](const int u, const int c, const CellInfo& ci, unsigned long long& run, float& low) {
+            // wv carries w_N^k of the pair: computed on the hardware sine and
+            // cosine where `direct`, otherwise advanced by the caller.
+            auto pair = [&](const int u, const int c, const CellInfo& ci, float2& wv, const bool direct,
+                            unsigned long long& run, float& low) {
                 if (!ci.valid) return;
                 int sa, ka1, sb, kb1, k2;
                 if (ci.ka != 0) {
@@ -406,8 +437,17 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const float2 Zp = buf[lpad<Q>(sb * Q + kb1)];
                 const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
                 const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
-                const float2 wv = cis_rev(-static_cast<float>(k) * invN);
+                if (direct) wv = cis_rev(-static_cast<float>(k) * invN);
                 const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
+                // k and M - k are both j = min(k, M - k) bins from the nearer end
+                // of the spectrum: past JT (and with every bin below F) both
+                // products are K' |X|^2, here K'/4 |2X|^2 (the same rounding)
+                if (F > M && min(k, km) >= sd.JT) {
+                    const float xr = Ar + pi, xi = Ai - pr, yr = Ar - pi, yi = Ai + pr;
+                    run = max(run, pack_nonneg(fmaf(xr, xr, xi * xi) * Kq, static_cast<unsigned>(k)));
+                    run = max(run, pack_nonneg(fmaf(yr, yr, yi * yi) * Kq, static_cast<unsigned>(km)));
+                    return;
+                }
                 if (k < F) from_x(k, 0.5f * (Ar + pi), 0.5f * (Ai - pr), ci, 0, run, low);
                 if (km < F && km != M) from_x(km, 0.5f * (Ar - pi), -0.5f * (Ai + pr), ci, 0, run, low);
                 if (ci.ka == 0 && u == 0 && M / 2 < F) {
@@ -424,6 +464,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 // Small sub-transforms: a thread's EPT elements are consecutive
                 // bins of one cell, visited in a rotated order that spreads the
                 // LDS banks, so each cell needs one cross-lane reduction per tile.
+                // (Mirror pairs here measured 4-7% slower for Q <= 256.)
                 constexpr int TPC = 2 * Q / EPT;   // threads per cell
                 const int c = tid / TPC;
                 const CellInfo ci = info[c];
@@ -457,8 +498,17 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                         }
                     } else {
                         if constexpr (GRP >= 2) {
+                            // pair u = tid + q NT has k = P u + k2: consecutive pairs
+                            // are Nfft / GRP bins apart, so w_N^k advances by
+                            // e^{-2 pi i / GRP}; g = 0 cells change slot (and k2)
+                            // at u = Q/2, i.e. q = GRP/4, where w_N^k restarts
+                            const float2 wstep = cis_rev(-1.0f / static_cast<float>(GRP));
+                            float2 wv = make_float2(1.f, 0.f);
 #pragma unroll 2
-                            for (int q = 0; q < GRP / 2; ++q) pair(tid + q * NT, c, ci, run, low);
+                            for (int q = 0; q < GRP / 2; ++q) {
+                                pair(tid + q * NT, c, ci, wv, q == 0 || q == GRP / 4, run, low);
+                                wv = cmul(wv, wstep);
+                            }
                         } else {
                             element((tid + gi * NT) % (2 * Q), c, ci, gi, run, low);
                         }
@@ -501,7 +551,6 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     const int k2 = half ? ci.kb : ci.ka;
                     const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
                     if (!(ci.valid && k2 >= 0 && k < F)) continue;
-                    ++mycnt;
                     const bool high = 4 * k > Nfft;
                     float pv;
                     if (mmp1 && !high) {
@@ -533,7 +582,6 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         if (tid < NCELL && F > M) {
             const CellInfo& ci = info[tid];
             if (ci.valid && ci.ka == 0) {
-                ++mycnt;
                 // E = 1 at Nyquist: |u|^4 / (2 E a^2 / K') = K' a^2 / 2 summed over channels
                 atomicMax(&best[tid], pack_best(0.5f * sd.Kp * accNyq, static_cast<unsigned>(M)));
             }

@@ -104,6 +104,28 @@ __device__ inline unsigned long long pack_best(float p, unsigned bin) {
     if (!(p > 0.0f)) p = 0.0f;   // NaN and negatives never win
     return (static_cast<unsigned long long>(__float_as_uint(p)) << 32) | (0xFFFFFFFFu - bin);
 }
+// pack_best of a value known to be >= 0 (a sum of squares times a positive
+// constant): the same bits without the clamp
+__device__ inline unsigned long long pack_nonneg(float p, unsigned bin) {
+    return (static_cast<unsigned long long>(__float_as_uint(p)) << 32) | (0xFFFFFFFFu - bin);
+}
+
+// A float array as a raw buffer resource (stride 0, 32-bit data format): a
+// load at a byte offset at or past `bytes` returns zero instead of touching
+// memory.  p and bytes must be the same in every lane of the wave: they are
+// taken from the first active lane, so that the resource sits in scalar
+// registers (a resource the compiler cannot prove uniform costs a loop over
+// lanes at every load).
+__device__ inline __amdgpu_buffer_rsrc_t buffer_of(const float* p, int bytes) {
+    const unsigned long long a = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(a));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(a >> 32));
+    float* u = reinterpret_cast<float*>((static_cast<unsigned long long>(hi) << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(u, 0, __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+__device__ inline float buffer_load(__amdgpu_buffer_rsrc_t r, unsigned byte_offset) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, static_cast<int>(byte_offset), 0, 0));
+}
 
 // e^{2 pi i cyc} from the hardware sine/cosine, whose argument is in
 // revolutions (absolute error 1.3e-7 for |cyc| <= 1.5, measured on gfx950)

@@ -1602,6 +1602,7 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
                 }
             }
             s.gA = static_cast<float>(M_PI / ((sc.s * d.fs) * (sc.s * d.fs)));
+            s.sgA = static_cast<float>(std::sqrt(M_PI * M_LOG2E) / (sc.s * d.fs));
             {
                 // the proposal's region bound: bins with E_j > 1e-4 take the low-bin seeds
                 const double alpha = sc.s * d.fs / sc.Nfft;

@@ -39,6 +39,7 @@ struct ScaleDev {
     // gH = L(E_JB), lL = L(E_1), gL = 1/sqrt(1 - E_1) (the largest over 1 <= j < JB)
     int JB;
     float gH, gL, lL;
+    float sgA;                // sqrt(gA log2 e): the window is exp2(-(sgA u)^2)
 };
 
 struct UnitDev {
