@@ -350,6 +350,19 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             // element(rem, c, ci, i, run): bin `rem` (of 2Q) of cell c; i indexes
             // the multichannel accumulators.
             // product of bin k from its spectrum value X = Xr + i Xi
+            // E-table entry of bin distance j < JT: from LDS, past kEtabLds
+            // (scales whose table does not fit there) from global memory
+            // through a buffer load, which the compiler cannot fold with the
+            // LDS read into one generic-address load
+            auto etab_at = [&](const int j) {
+                float2 t = etl[min(j, kEtabLds)];
+                if (sd.JT > kEtabLds && j >= kEtabLds) {
+                    const __amdgpu_buffer_rsrc_t er = buffer_of(reinterpret_cast<const float*>(sd.etab), sd.JT * 8);
+                    t = make_float2(buffer_load(er, static_cast<unsigned>(j) * 8u),
+                                    buffer_load(er, static_cast<unsigned>(j) * 8u + 4u));
+                }
+                return t;
+            };
             auto from_x = [&](const int k, const float Xr, const float Xi, const CellInfo& ci, const int i,
                               unsigned long long& run, float& low) {
                 const bool high = 4 * k > Nfft;
@@ -373,8 +386,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 }
                 float pe = num * sd.Kp;   // the product where E = 0
                 if (edge) {
-                    float2 tt = etl[min(j, kEtabLds)];
-                    if (j >= kEtabLds) tt = sd.etab[j];
+                    const float2 tt = etab_at(j);
                     const float den = fmaf(tt.x, num, tt.y * a * a);
                     pe = den > 0.0f ? num * num * __builtin_amdgcn_rcpf(den) : 0.0f;
                     if constexpr (LB)
@@ -439,17 +451,49 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
                 if (direct) wv = cis_rev(-static_cast<float>(k) * invN);
                 const float pr = wv.x * Br - wv.y * Bi, pi = wv.x * Bi + wv.y * Br;
+                // 2 X[k] and 2 X[M - k]: the factor 2 is exact, and every
+                // product below is formed with the rounding of the unscaled one
+                const float xr = Ar + pi, xi = Ai - pr, yr = Ar - pi, yi = -(Ai + pr);
+                const float nx = fmaf(xr, xr, xi * xi), ny = fmaf(yr, yr, yi * yi);   // 4 |X|^2
                 // k and M - k are both j = min(k, M - k) bins from the nearer end
-                // of the spectrum: past JT (and with every bin below F) both
-                // products are K' |X|^2, here K'/4 |2X|^2 (the same rounding)
-                if (F > M && min(k, km) >= sd.JT) {
-                    const float xr = Ar + pi, xi = Ai - pr, yr = Ar - pi, yi = Ai + pr;
-                    run = max(run, pack_nonneg(fmaf(xr, xr, xi * xi) * Kq, static_cast<unsigned>(k)));
-                    run = max(run, pack_nonneg(fmaf(yr, yr, yi * yi) * Kq, static_cast<unsigned>(km)));
-                    return;
+                // of the spectrum
+                const int j = min(k, km);
+                if (j >= sd.JT) {
+                    // E = 0 for both: P = K' |X|^2 = K'/4 |2X|^2
+                    if (F > M) {
+                        run = max(run, pack_nonneg(nx * Kq, static_cast<unsigned>(k)));
+                        run = max(run, pack_nonneg(ny * Kq, static_cast<unsigned>(km)));
+                    } else {
+                        if (k < F) run = max(run, pack_nonneg(nx * Kq, static_cast<unsigned>(k)));
+                        if (km < F) run = max(run, pack_nonneg(ny * Kq, static_cast<unsigned>(km)));
+                    }
+                } else {
+                    // E > 0: P = |X|^4 / ((1 - E)/K' |X|^2 + 2E/K' a^2), a = Re(X e^{2 pi i cyc}).
+                    // The low bin of the pair has cyc = j d / Nfft, the high one
+                    // -cyc - (iL odd) / 2, so its a is +-Re(X conj e^{2 pi i cyc}):
+                    // one sine and cosine and one E-table entry serve both bins,
+                    // and the sign drops out of a^2.
+                    // j, dint < 2^24: the 24-bit product is exact modulo 2^32, and Nfft divides 2^32
+                    const unsigned ci_int = __umul24(static_cast<unsigned>(j), static_cast<unsigned>(ci.dint)) &
+                                            static_cast<unsigned>(Nfft - 1);
+                    const float cyc = static_cast<float>(ci_int) * invN + static_cast<float>(j) * ci.dfrac * invN;
+                    float sn, cs;
+                    twiddle_rev(cyc, &sn, &cs);
+                    const float2 tt = etab_at(j);
+                    const float sg = k < km ? -sn : sn;   // k is the low bin when k < M - k
+                    const float ak = fmaf(xi, sg, xr * cs), am = fmaf(-yi, sg, yr * cs);
+                    const float dk = fmaf(tt.x, nx, tt.y * ak * ak), dm = fmaf(tt.x, ny, tt.y * am * am);
+                    const float pk = dk > 0.0f ? 0.25f * (nx * nx * __builtin_amdgcn_rcpf(dk)) : 0.0f;
+                    const float pm = dm > 0.0f ? 0.25f * (ny * ny * __builtin_amdgcn_rcpf(dm)) : 0.0f;
+                    const bool inK = k < F, inM = km < F && km != M;   // km = M: the DC bin's mirror
+                    if (inK) run = max(run, pack_best(pk, static_cast<unsigned>(k)));
+                    if (inM) run = max(run, pack_best(pm, static_cast<unsigned>(km)));
+                    if constexpr (LB)
+                        if (j >= 1 && j < sd.JB) {
+                            if (inK) low = fmaxf(low, nx * Kq);
+                            if (inM) low = fmaxf(low, ny * Kq);
+                        }
                 }
-                if (k < F) from_x(k, 0.5f * (Ar + pi), 0.5f * (Ai - pr), ci, 0, run, low);
-                if (km < F && km != M) from_x(km, 0.5f * (Ar - pi), -0.5f * (Ai + pr), ci, 0, run, low);
                 if (ci.ka == 0 && u == 0 && M / 2 < F) {
                     // (a, Q/2): bin M/2, its own mirror (Z[k] = Z[M-k])
                     const float2 Zh = buf[lpad<Q>(2 * c * Q + Q / 2)];
@@ -461,18 +505,19 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             };
             constexpr bool CONTIG = !MULTI && Q >= 8 && Q <= 256;
             if constexpr (CONTIG) {
-                // Small sub-transforms: a thread's EPT elements are consecutive
-                // bins of one cell, visited in a rotated order that spreads the
-                // LDS banks, so each cell needs one cross-lane reduction per tile.
-                // (Mirror pairs here measured 4-7% slower for Q <= 256.)
+                // Small sub-transforms: a thread's EPT / 2 mirror pairs are
+                // consecutive pairs of one cell, visited in a rotated order that
+                // spreads the LDS banks, so each cell needs one cross-lane
+                // reduction per tile.
                 constexpr int TPC = 2 * Q / EPT;   // threads per cell
                 const int c = tid / TPC;
                 const CellInfo ci = info[c];
-                const int r0 = (tid % TPC) * EPT;
+                const int u0 = (tid % TPC) * (EPT / 2);
                 unsigned long long run = 0ull;
                 float low = 0.0f;
+                float2 wv;
 #pragma unroll 2
-                for (int s2 = 0; s2 < EPT; ++s2) element(r0 + ((s2 + tid) & (EPT - 1)), c, ci, 0, run, low);
+                for (int s2 = 0; s2 < EPT / 2; ++s2) pair(u0 + ((s2 + tid) & (EPT / 2 - 1)), c, ci, wv, true, run, low);
 #pragma unroll
                 for (int off = TPC / 2; off >= 1; off >>= 1) run = max(run, shfl_xor_u64(run, off));
                 if ((tid & (TPC - 1)) == 0 && run) atomicMax(&best[c], run);
