@@ -283,6 +283,15 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 return make_float2(x0 * __builtin_amdgcn_exp2f(-(s0 * s0)), x1 * __builtin_amdgcn_exp2f(-(s1 * s1)));
             };
             constexpr int GL = Q >= NT ? Q / NT : 1;   // consecutive pairs sharing a cell
+            // One cell per time position (G = 1, P <= 2): every cell has k2 =
+            // 0 and P/2, and for GL = 1 a thread's pair index m = tid mod Q
+            // is the same in all its cells, so the slot-b pre-twiddle is
+            // computed once per tile
+            const bool g1 = GL == 1 && __builtin_amdgcn_readfirstlane(G) == 1;
+            float2 tw1 = make_float2(1.f, 0.f);
+            if (g1)
+                tw1 = cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(P / 2), static_cast<unsigned>(tid % Q))) *
+                              invM);
 #pragma unroll
             for (int gi = 0; gi < EPT / 2 / GL; ++gi) {
                 const int c = (tid + gi * GL * NT) / Q;
@@ -295,8 +304,11 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 // apart in one cell, so later ones advance by w_M^{k2 NT}
                 const unsigned um0 = static_cast<unsigned>(m0);
                 const unsigned uka = static_cast<unsigned>(ci.ka), ukb = static_cast<unsigned>(max(ci.kb, 0));
-                float2 twa = cis_rev(-static_cast<float>(__umul24(uka, um0)) * invM);
-                float2 twb = cis_rev(-static_cast<float>(__umul24(ukb, um0)) * invM);
+                float2 twa = make_float2(1.f, 0.f), twb = tw1;
+                if (!g1) {
+                    twa = cis_rev(-static_cast<float>(__umul24(uka, um0)) * invM);
+                    twb = cis_rev(-static_cast<float>(__umul24(ukb, um0)) * invM);
+                }
                 float2 sta = make_float2(1.f, 0.f), stb = make_float2(1.f, 0.f);
                 if constexpr (GL > 1) {
                     sta = cis_rev(-static_cast<float>(__umul24(uka, static_cast<unsigned>(NT))) * invM);
