@@ -325,10 +325,20 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                         if constexpr (Q == kMaxSubTransform) {
                             // input m + f Q enters sub-transform k2 with e^{-2 pi i k2 f / P}
                             if (sd.fold > 1) {
+                                // e^{-2 pi i k2 f / P} by products of e^{-2 pi i k2 / P},
+                                // restarted every 16 sections from the exact revolution
+                                // count (k2 f mod P) / P, so the error stays that of a
+                                // few products however many sections a window spans
                                 const float2 ta = ftw[2 * c], tb = ftw[2 * c + 1];
+                                const float invP = 1.0f / static_cast<float>(P);   // P is a power of two
                                 float2 wa = ta, wb = tb, sa = z, sb = z;
                                 for (int f = 1; f < sd.fold; ++f) {
-                                    if (f > 1) {
+                                    if ((f & 15) == 0) {
+                                        const unsigned uf = static_cast<unsigned>(f), mask = static_cast<unsigned>(P - 1);
+                                        wa = cis_rev(-static_cast<float>((uka * uf) & mask) * invP);
+                                        wb = ci.kb >= 0 ? cis_rev(-static_cast<float>((ukb * uf) & mask) * invP)
+                                                        : make_float2(0.f, 0.f);
+                                    } else if (f > 1) {
                                         wa = cmul(wa, ta);
                                         wb = cmul(wb, tb);
                                     }

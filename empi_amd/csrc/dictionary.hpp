@@ -18,11 +18,16 @@ constexpr double kGaussHalfWidth = 3.0;        // src/gabor.cpp:18
 constexpr double kMinScaleInSamples = 2.0;     // src/gabor.cpp:23
 // Sub-transforms of the column kernels (complex points): up to 4096 for
 // single-channel units, 8192 for multichannel ones (their channel loop keeps
-// one workgroup per CU either way).  Longer windows fold onto 4096 points, at
-// most 32 sections: windows up to 2 * 4096 * 32 = 262,144 samples.
+// one workgroup per CU either way).  Longer windows fold onto 4096 points,
+// one section per 8192 samples.
 constexpr int kMaxSubTransform = 4096;
 constexpr int kMaxSubTransformMulti = 8192;
-constexpr int kMaxFold = 32;
+// Sizes the column kernels compute exactly: bin indices, window offsets and
+// the revolution counts of their twiddles are integers below 2^24 (24-bit
+// products, exact FP32 conversions), so FFTs up to 2^25 points and signals
+// up to 2^24 samples.
+constexpr int kMaxNfft = 1 << 25;
+constexpr int kMaxSamples = 1 << 24;
 
 struct Scale {
     double s;        // scale [s]

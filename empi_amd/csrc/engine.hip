@@ -1465,11 +1465,12 @@ empi_status dict_for(empi_plan* p, const empi_dict_params& dp, double fs, int N,
             return EMPI_OK;
         }
     }
+    if (N > kMaxSamples) return EMPI_UNSUPPORTED;
     Dictionary d;
     empi_status st = build_dictionary(dp, fs, N, &d, max_q);
     if (st != EMPI_OK) return st;
     for (const Scale& sc : d.scales) {
-        if (sc.Q > kMaxQ || sc.fold > kMaxFold) return EMPI_UNSUPPORTED;
+        if (sc.Q > kMaxQ || sc.Nfft > kMaxNfft) return EMPI_UNSUPPORTED;
     }
     *id = static_cast<int>(p->dicts.size());
     p->dicts.push_back(std::move(d));

@@ -35,7 +35,7 @@ enum {
     EMPI_NO_MEMORY = 5,              /* "insufficientMemory"  src/empi.cpp:189-190 */
     EMPI_INTERNAL = 6,               /* "internalError"       src/empi.cpp:191-192 */
     EMPI_DEVICE = 7,                 /* "deviceError": a HIP runtime call failed */
-    EMPI_UNSUPPORTED = 8             /* "unsupportedSize": a window beyond 262,144 samples, or an unsupported proposal batch */
+    EMPI_UNSUPPORTED = 8             /* "unsupportedSize": a signal beyond 2^24 samples or an FFT beyond 2^25 points, or an unsupported proposal batch */
 };
 
 /* Decomposition variants (src/empi.cpp:114-130). */

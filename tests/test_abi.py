@@ -67,8 +67,7 @@ def test_plan_create_rejects_bad_items_before_any_device_call():
         with pytest.raises(E.EmpiError) as ei:
             E.Plan([spec])
         assert ei.value.status == 4, spec
-    # a window beyond 262,144 samples needs more than 32 folds of a 4096-point
-    # sub-transform
+    # beyond 2^24 samples the kernels' 24-bit index arithmetic is not exact
     with pytest.raises(E.EmpiError) as ei:
-        E.Plan([E.ItemSpec(1, 300000, 256.0, E.SMP, 10, 99.0, 0.01)])
+        E.Plan([E.ItemSpec(1, (1 << 24) + 1, 256.0, E.SMP, 10, 99.0, 0.01)])
     assert ei.value.status == 8

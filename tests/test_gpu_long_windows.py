@@ -3,8 +3,8 @@
 onto one sub-transform (input m1 + 4096 f enters sub-transform k2 with
 e^{-2 pi i k2 f / P}); multichannel units fold beyond 16,384 samples.  The
 largest scales of a 16,384-sample signal fold twice, of a 32,768-sample
-signal four times; every class runs in the same plan.  Parity against the
-FP64 oracle, as for the short signals."""
+signal four times, of a 300,000-sample signal 37 times; every class runs in
+the same plan.  Parity against the FP64 oracle, as for the short signals."""
 import pytest
 
 import empi_amd as E
@@ -53,3 +53,17 @@ def test_nine_folds():
     assert res.status == 0
     ora = _oracle(x[None, :], 256.0, 0.01, po.SMP, 2, 99.0)
     _check_item(res, ora, E.SMP, "70000 samples")
+
+
+def test_thirty_seven_folds():
+    """300,000 samples at 64 Hz (energyError 0.1 keeps the oracle's dictionary
+    small): the largest windows span 150,000 packed points, 37 sections,
+    past the 16-section restarts of the section twiddles."""
+    _announce("long windows: 300000 samples @64 Hz, eps 0.1, 3 iterations (37 folds)")
+    x = synth.pink_with_atoms(4360, 300000, 64.0, 4)
+    spec = E.ItemSpec(1, 300000, 64.0, E.SMP, 3, 99.0, 0.1)
+    plan = _run_plan([spec], [x])
+    res = plan.result(0)
+    assert res.status == 0
+    ora = _oracle(x[None, :], 64.0, 0.1, po.SMP, 3, 99.0)
+    _check_item(res, ora, E.SMP, "300000 samples", min_compared=3)
