@@ -37,6 +37,7 @@ EXPORTS = [
     "empi_plan_tuning_size", "empi_plan_autotune", "empi_plan_get_tuning", "empi_plan_set_tuning",
     "empi_proposal_feature_dim", "empi_proposal_bound_factor", "empi_plan_set_proposal", "empi_plan_proposal_stats",
     "empi_plan_set_training_capacity", "empi_plan_training_rows", "empi_decompose_noalloc", "empi_plan_stage",
+    "empi_debug_argmax",
 ]
 
 PROPOSAL_OFF, PROPOSAL_ON, PROPOSAL_COLLECT = 0, 1, 2
@@ -133,6 +134,8 @@ def lib():
         L.empi_plan_training_rows.argtypes = [vp, fp, fp, C.c_size_t, C.POINTER(C.c_size_t)]
         L.empi_decompose_noalloc.argtypes = [C.POINTER(Item), vp, C.c_int, C.POINTER(Atom), C.POINTER(C.c_int),
                                              dp, vp]
+        ip = C.POINTER(C.c_int)
+        L.empi_debug_argmax.argtypes = [C.c_int, ip, ip, ip, ip, ip, fp, ip, fp]
         L.empi_strerror.argtypes = [C.c_int]
         L.empi_strerror.restype = C.c_char_p
         L.empi_version.restype = C.c_char_p
@@ -181,6 +184,27 @@ class ItemSpec:
         return Item(self.channels, self.samples, float(self.fs), int(self.variant),
                     int(self.iteration_max), float(self.energy_percent),
                     dict_params(self.energy_error, self.scale_min, self.scale_max, self.freq_max))
+
+
+def debug_argmax(maps):
+    """Test hook: the engine's argmax keys over planted maps on the device.
+    maps: list of (products [T][G][B] float32, bins [G][B] int, cells per
+    tile).  Returns ((scale, bin, time), best product)."""
+    L = len(maps)
+    T = np.array([m[0].shape[0] for m in maps], np.int32)
+    G = np.array([m[0].shape[1] for m in maps], np.int32)
+    B = np.array([m[0].shape[2] for m in maps], np.int32)
+    nc = np.array([m[2] for m in maps], np.int32)
+    bins = np.ascontiguousarray(np.concatenate([np.asarray(m[1], np.int32).ravel() for m in maps]))
+    prods = np.ascontiguousarray(np.concatenate([np.asarray(m[0], np.float32).ravel() for m in maps]))
+    out = np.zeros(3, np.int32)
+    best = np.zeros(1, np.float32)
+    ip = C.POINTER(C.c_int)
+    _check(lib().empi_debug_argmax(L, T.ctypes.data_as(ip), G.ctypes.data_as(ip), nc.ctypes.data_as(ip),
+                                   B.ctypes.data_as(ip), bins.ctypes.data_as(ip),
+                                   prods.ctypes.data_as(C.POINTER(C.c_float)), out.ctypes.data_as(ip),
+                                   best.ctypes.data_as(C.POINTER(C.c_float))))
+    return tuple(int(v) for v in out), float(best[0])
 
 
 def decompose_noalloc(item, signal=None, device_ptr=None, stream=None) -> ItemResult:

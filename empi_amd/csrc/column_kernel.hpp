@@ -160,6 +160,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     constexpr int SEG = 2 * Q < 64 ? 2 * Q : 64;          // lanes sharing a cell
     constexpr int GRP = 2 * Q >= NT ? 2 * Q / NT : 1;      // consecutive elements sharing a cell
     static_assert(EPT % GRP == 0, "element groups");
+    static_assert(NCELL <= (1 << kTileCellBits), "cell index of a tile-best key");
     __shared__ float2 buf[Lds<Q, NPTS>::SIZE];
     __shared__ CellInfo info[NCELL];
     __shared__ unsigned long long best[NCELL];
@@ -668,9 +669,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             } else {
                 cb = cellbest[job.cell_off + cell0 + c];
             }
-            const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(cb);
-            const unsigned key = (bin << 10) | static_cast<unsigned>(c);
-            tb = max(tb, (cb & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFu - key));
+            tb = max(tb, tile_key(cb, c, kTileCellBits));
         }
         if (tb) atomicMax(&tbest, tb);
         __syncthreads();

@@ -110,6 +110,36 @@ __device__ inline unsigned long long pack_nonneg(float p, unsigned bin) {
     return (static_cast<unsigned long long>(__float_as_uint(p)) << 32) | (0xFFFFFFFFu - bin);
 }
 
+// The argmax keys, shared by the column kernels, the selection and the
+// argmax test hook.  Every level orders by product, and exact ties by the
+// smallest scale, then bin, then time index:
+//   cell best  (product bits << 32) | ~bin                   (pack_best)
+//   tile best  (product bits << 32) | ~(bin << cell_bits | cell in tile)
+//   selection  product desc, then (scale << 52 | bin << 26 | time) asc
+__device__ inline unsigned long long tile_key(unsigned long long cell_best, int cell, int cell_bits) {
+    const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(cell_best);
+    const unsigned key = (bin << cell_bits) | static_cast<unsigned>(cell);
+    return (cell_best & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFu - key);
+}
+struct SelBest {
+    float p;
+    unsigned long long key;   // (scale << 52) | (bin << 26) | t
+};
+__device__ inline bool sel_better(const SelBest& a, const SelBest& b) {
+    return a.p > b.p || (a.p == b.p && a.key < b.key);
+}
+__device__ inline SelBest sel_key(float p, int scale, unsigned bin, int t) {
+    return SelBest{p, (static_cast<unsigned long long>(scale) << 52) | (static_cast<unsigned long long>(bin) << 26) |
+                          static_cast<unsigned long long>(t)};
+}
+// candidate of tile `tile` (ncell cells, G per time index) of scale `scale`
+__device__ inline SelBest sel_of_tile(unsigned long long tile_best, int scale, int tile, int ncell, int G,
+                                      int cell_bits) {
+    const unsigned key = 0xFFFFFFFFu - static_cast<unsigned>(tile_best & 0xFFFFFFFFu);
+    const int cell = tile * ncell + static_cast<int>(key & ((1u << cell_bits) - 1u));
+    return sel_key(__uint_as_float(static_cast<unsigned>(tile_best >> 32)), scale, key >> cell_bits, cell / G);
+}
+
 // A float array as a raw buffer resource (stride 0, 32-bit data format): a
 // load at a byte offset at or past `bytes` returns zero instead of touching
 // memory.  p and bytes must be the same in every lane of the wave: they are

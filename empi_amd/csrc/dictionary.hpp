@@ -24,9 +24,9 @@ constexpr int kMaxSubTransform = 4096;
 constexpr int kMaxSubTransformMulti = 8192;
 // Sizes the column kernels compute exactly: bin indices, window offsets and
 // the revolution counts of their twiddles are integers below 2^24 (24-bit
-// products, exact FP32 conversions), so FFTs up to 2^25 points and signals
-// up to 2^24 samples.
-constexpr int kMaxNfft = 1 << 25;
+// products, exact FP32 conversions, bins in 24 bits of a tile-best key), so
+// FFTs up to 2^24 points and signals up to 2^24 samples.
+constexpr int kMaxNfft = 1 << 24;
 constexpr int kMaxSamples = 1 << 24;
 
 struct Scale {

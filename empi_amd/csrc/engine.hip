@@ -360,14 +360,6 @@ struct RegionState {
     int pad;
 };
 
-struct SelBest {
-    float p;
-    unsigned long long key;   // (scale << 52) | (bin << 26) | t
-};
-__device__ inline bool sel_better(const SelBest& a, const SelBest& b) {
-    return a.p > b.p || (a.p == b.p && a.key < b.key);
-}
-
 // mode bit 0: find the best atom; bit 1: subtract it and test the stop rule.
 // split: multichannel units only record the chosen entry (sel_*); k_fit_mc,
 // k_params_mc, k_sub_mc and k_stop_mc finish their iteration one workgroup
@@ -399,11 +391,8 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
             if ((R.flags & 1) || R.rbest == 0ull) continue;
             const int l = r / ud.B;
             const int G = scales[ud.scale0 + l].G;
-            SelBest c;
-            c.p = __uint_as_float(static_cast<unsigned>(R.rbest >> 32));
-            const unsigned bin = 0xFFFFFFFFu - static_cast<unsigned>(R.rbest & 0xFFFFFFFFu);
-            c.key = (static_cast<unsigned long long>(l) << 52) | (static_cast<unsigned long long>(bin) << 26) |
-                    static_cast<unsigned long long>(R.rcell / G);
+            const SelBest c = sel_key(__uint_as_float(static_cast<unsigned>(R.rbest >> 32)), l,
+                                      0xFFFFFFFFu - static_cast<unsigned>(R.rbest & 0xFFFFFFFFu), R.rcell / G);
             if (sel_better(c, b)) b = c;
         }
     }
@@ -411,14 +400,7 @@ __global__ __launch_bounds__(NT) void k_select(const UnitDev* __restrict__ units
         const UnitScale us = uscales[ud.us_off + l];
         const int G = scales[ud.scale0 + l].G;
         for (int i = tid; i < us.tiles; i += NT) {
-            const unsigned long long v = tilebest[us.tile_off + i];
-            SelBest c;
-            c.p = __uint_as_float(static_cast<unsigned>(v >> 32));
-            const unsigned key = 0xFFFFFFFFu - static_cast<unsigned>(v & 0xFFFFFFFFu);
-            const unsigned bin = key >> 10;
-            const int cell = i * us.ncell + static_cast<int>(key & 1023u);
-            c.key = (static_cast<unsigned long long>(l) << 52) | (static_cast<unsigned long long>(bin) << 26) |
-                    static_cast<unsigned long long>(cell / G);
+            const SelBest c = sel_of_tile(tilebest[us.tile_off + i], l, i, us.ncell, G, kTileCellBits);
             if (sel_better(c, b)) b = c;
         }
     }
@@ -2163,6 +2145,59 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
 
 // ------------------------------------------------------------------- C ABI
 
+// The argmax keys over a planted map: per cell the packed bin keys, per tile
+// the tile keys, and the selection over tiles, in one workgroup (a test hook:
+// planted maps are small).  Products [scale][t][g][b], bins [scale][g][b].
+template <int NT>
+__global__ __launch_bounds__(NT) void k_debug_argmax(int L, const int* __restrict__ T, const int* __restrict__ G,
+                                                     const int* __restrict__ ncell, const int* __restrict__ B,
+                                                     const long long* __restrict__ poff,
+                                                     const long long* __restrict__ boff,
+                                                     const float* __restrict__ products,
+                                                     const int* __restrict__ bins, int* __restrict__ out,
+                                                     float* __restrict__ best) {
+    __shared__ float redp[NT];
+    __shared__ unsigned long long redk[NT];
+    const int tid = threadIdx.x;
+    SelBest b{-1.0f, ~0ull};
+    for (int l = 0; l < L; ++l) {
+        const int cells = T[l] * G[l], nc = ncell[l];
+        const int tiles = (cells + nc - 1) / nc;
+        for (int i = tid; i < tiles; i += NT) {
+            unsigned long long tb = 0ull;
+            for (int c = 0; c < nc && i * nc + c < cells; ++c) {
+                const int cg = i * nc + c, g = cg % G[l];
+                unsigned long long cb = 0ull;
+                for (int q = 0; q < B[l]; ++q)
+                    cb = max(cb, pack_best(products[poff[l] + static_cast<long long>(cg) * B[l] + q],
+                                           static_cast<unsigned>(bins[boff[l] + g * B[l] + q])));
+                tb = max(tb, tile_key(cb, c, kTileCellBits));
+            }
+            const SelBest cand = sel_of_tile(tb, l, i, nc, G[l], kTileCellBits);
+            if (sel_better(cand, b)) b = cand;
+        }
+    }
+    redp[tid] = b.p;
+    redk[tid] = b.key;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            const SelBest o{redp[tid + s], redk[tid + s]}, m{redp[tid], redk[tid]};
+            if (sel_better(o, m)) {
+                redp[tid] = o.p;
+                redk[tid] = o.key;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[0] = static_cast<int>(redk[0] >> 52);
+        out[1] = static_cast<int>((redk[0] >> 26) & ((1ull << 26) - 1));
+        out[2] = static_cast<int>(redk[0] & ((1ull << 26) - 1));
+        *best = redp[0];
+    }
+}
+
 extern "C" {
 
 const char* empi_strerror(int status) {
@@ -2476,6 +2511,46 @@ empi_status empi_plan_training_rows(empi_plan* p, float* X, float* target, size_
     if (n && target) EMPI_HIP(hipMemcpy(target, p->d_trainT.p, sizeof(float) * n, hipMemcpyDeviceToHost));
     EMPI_HIP(hipMemset(p->d_traincount.p, 0, sizeof(unsigned long long)));
     *rows = static_cast<size_t>(n);
+    return EMPI_OK;
+}
+
+// ----------------------------------------------------------- argmax test hook
+
+empi_status empi_debug_argmax(int scales, const int* T, const int* G, const int* ncell, const int* bins_per_cell,
+                              const int* bins, const float* products, int out[3], float* best) {
+    if (scales <= 0 || !T || !G || !ncell || !bins_per_cell || !bins || !products || !out || !best)
+        return EMPI_INVALID_ARGUMENT;
+    std::vector<long long> poff(scales + 1, 0), boff(scales + 1, 0);
+    for (int l = 0; l < scales; ++l) {
+        if (T[l] <= 0 || G[l] <= 0 || bins_per_cell[l] <= 0 || ncell[l] <= 0 || ncell[l] > (1 << kTileCellBits))
+            return EMPI_INVALID_ARGUMENT;
+        for (int q = 0; q < G[l] * bins_per_cell[l]; ++q) {
+            const int k = bins[boff[l] + q];
+            if (k < 0 || k >= (1 << (32 - kTileCellBits))) return EMPI_INVALID_ARGUMENT;
+        }
+        poff[l + 1] = poff[l] + static_cast<long long>(T[l]) * G[l] * bins_per_cell[l];
+        boff[l + 1] = boff[l] + static_cast<long long>(G[l]) * bins_per_cell[l];
+        if (T[l] >= (1 << 26)) return EMPI_INVALID_ARGUMENT;
+    }
+    DevBuf<int> dT, dG, dN, dB, dbins, dout;
+    DevBuf<long long> dpo, dbo;
+    DevBuf<float> dp, dbest;
+    empi_status st;
+    if ((st = dT.alloc(scales)) || (st = dG.alloc(scales)) || (st = dN.alloc(scales)) || (st = dB.alloc(scales)) ||
+        (st = dbins.alloc(boff[scales])) || (st = dout.alloc(3)) || (st = dpo.alloc(scales)) ||
+        (st = dbo.alloc(scales)) || (st = dp.alloc(poff[scales])) || (st = dbest.alloc(1)))
+        return st;
+    hipStream_t s = nullptr;
+    if ((st = dT.upload(T, scales, s)) || (st = dG.upload(G, scales, s)) || (st = dN.upload(ncell, scales, s)) ||
+        (st = dB.upload(bins_per_cell, scales, s)) || (st = dbins.upload(bins, boff[scales], s)) ||
+        (st = dpo.upload(poff.data(), scales, s)) || (st = dbo.upload(boff.data(), scales, s)) ||
+        (st = dp.upload(products, poff[scales], s)))
+        return st;
+    hipLaunchKernelGGL(k_debug_argmax<256>, dim3(1), dim3(256), 0, s, scales, dT.p, dG.p, dN.p, dB.p, dpo.p, dbo.p,
+                       dp.p, dbins.p, dout.p, dbest.p);
+    EMPI_HIP(hipGetLastError());
+    EMPI_HIP(hipMemcpy(out, dout.p, 3 * sizeof(int), hipMemcpyDeviceToHost));
+    EMPI_HIP(hipMemcpy(best, dbest.p, sizeof(float), hipMemcpyDeviceToHost));
     return EMPI_OK;
 }
 

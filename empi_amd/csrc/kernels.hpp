@@ -5,7 +5,7 @@
 //   cellbest   u64  [unit][scale][t][g]            best product of one cell
 //              (high 32 bits: FP32 product bits, low 32 bits: ~bin)
 //   tilebest   u64  [unit][scale][tile]            best of a workgroup's cells
-//              (high: product bits, low: ~(bin << 10 | cell in tile))
+//              (high: product bits, low: ~(bin << kTileCellBits | cell in tile))
 //   atoms      empi_atom [item][channel][iteration]
 //   energies   FP64 [item][row][iteration + 1]
 // A "unit" is one MP loop: one channel of an SMP item, the channel sum of an
@@ -19,6 +19,10 @@
 #include "empi_hip.h"
 
 namespace empi {
+
+// Low bits of a tile-best key holding the cell within the tile (at most 256
+// cells per tile); the bin takes the other 24 bits.
+constexpr int kTileCellBits = 8;
 
 struct ScaleDev {
     double s, fs, tMax;

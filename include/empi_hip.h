@@ -35,7 +35,7 @@ enum {
     EMPI_NO_MEMORY = 5,              /* "insufficientMemory"  src/empi.cpp:189-190 */
     EMPI_INTERNAL = 6,               /* "internalError"       src/empi.cpp:191-192 */
     EMPI_DEVICE = 7,                 /* "deviceError": a HIP runtime call failed */
-    EMPI_UNSUPPORTED = 8             /* "unsupportedSize": a signal beyond 2^24 samples or an FFT beyond 2^25 points, or an unsupported proposal batch */
+    EMPI_UNSUPPORTED = 8             /* "unsupportedSize": a signal beyond 2^24 samples or an FFT beyond 2^24 points, or an unsupported proposal batch */
 };
 
 /* Decomposition variants (src/empi.cpp:114-130). */
@@ -267,6 +267,19 @@ empi_status empi_plan_proposal_stats(const empi_plan* plan, unsigned long long o
  * the number copied; reading resets the store. */
 empi_status empi_plan_set_training_capacity(empi_plan* plan, size_t rows, unsigned stride);
 empi_status empi_plan_training_rows(empi_plan* plan, float* X, float* target, size_t capacity, size_t* rows);
+
+/* Test hook (no counterpart in the reference; for tests of the argmax order):
+ * runs the engine's argmax keys -- per cell the packed (product, bin) key,
+ * per tile the (product, bin, cell) key, and the selection's (product,
+ * scale, bin, time) order -- over a planted map on the device.  scales maps;
+ * map l has T[l] time indices, G[l] cells per time index, ncell[l] cells per
+ * tile (at most 256) and bins_per_cell[l] bins per cell; bins holds, map
+ * after map, [G][bins_per_cell] bin indices (below 2^24), products, map after
+ * map, [T][G][bins_per_cell] FP32 products.  out receives (scale, bin, time)
+ * of the maximum product, exact ties going to the smallest scale, then bin,
+ * then time, and *best that product (zero when no product is positive). */
+empi_status empi_debug_argmax(int scales, const int* T, const int* G, const int* ncell, const int* bins_per_cell,
+                              const int* bins, const float* products, int out[3], float* best);
 
 /* Camel-case reference error text for a status code. */
 const char* empi_strerror(int status);
