@@ -1299,6 +1299,24 @@ int cells_per_wg(int Q) { return class_slots(Q) / 2; }
 
 }  // namespace
 
+// iterations the host may queue past the last one it has seen finish, and
+// the ring of active-unit counts it reads them through
+constexpr int kPollLag = 2;
+constexpr int kPollRing = 4;
+
+// number of active units, for the host's end-of-loop test
+__global__ __launch_bounds__(256) void k_count_active(const UnitState* __restrict__ state, int U,
+                                                      int* __restrict__ out) {
+    __shared__ int n;
+    if (threadIdx.x == 0) n = 0;
+    __syncthreads();
+    int c = 0;
+    for (int u = threadIdx.x; u < U; u += 256) c += state[u].active != 0;
+    if (c) atomicAdd(&n, c);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = n;
+}
+
 struct empi_plan {
     std::vector<ItemInfo> items;
     std::vector<Dictionary> dicts;
@@ -1385,6 +1403,11 @@ struct empi_plan {
     hipStream_t upload = nullptr;
     hipGraphExec_t iter_exec = nullptr;
     hipGraph_t iter_graph = nullptr;
+    // end-of-loop detection: after each iteration the number of active units
+    // goes to a pinned ring slot with an event (allocating runs only)
+    DevBuf<int> d_nactive;
+    int* h_nactive = nullptr;
+    hipEvent_t poll_ev[kPollRing] = {};
 
     // optional per-class kernel timing (HIP events on the launch stream)
     bool profiling = false;
@@ -1411,6 +1434,9 @@ struct empi_plan {
         if (prop_graph) (void)hipGraphDestroy(prop_graph);
         if (stream && !noalloc) (void)hipStreamDestroy(stream);
         if (upload) (void)hipStreamDestroy(upload);
+        if (h_nactive) (void)hipHostFree(h_nactive);
+        for (hipEvent_t e : poll_ev)
+            if (e) (void)hipEventDestroy(e);
         for (auto& v : ev)
             for (auto& e : v) {
                 (void)hipEventDestroy(e.first);
@@ -2084,22 +2110,49 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
         p->iter_graph = g;
         EMPI_HIP(hipGraphInstantiate(&p->iter_exec, g, nullptr, nullptr, 0));
     }
+    // End of the loop: an allocating run queues at most kPollLag iterations
+    // past the last one whose active-unit count it has read (so the device
+    // never waits for the host, and at most kPollLag empty iterations follow
+    // the last stop); an allocation-free run reads the unit states every 8
+    // iterations through pageable memory.
+    const bool ring = !p->noalloc;
+    if (ring) {
+        if (!p->d_nactive.p && (st = p->d_nactive.alloc(1)) != EMPI_OK) return st;
+        if (!p->h_nactive) {
+            EMPI_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->h_nactive), sizeof(int) * kPollRing,
+                                   hipHostMallocDefault));
+            for (hipEvent_t& e : p->poll_ev) EMPI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+    }
     for (int it = 0; it < p->maxIter; ++it) {
+        if (ring && it >= kPollLag) {
+            const int slot = (it - kPollLag) % kPollRing;
+            EMPI_HIP(hipEventSynchronize(p->poll_ev[slot]));
+            if (p->h_nactive[slot] == 0) break;
+        }
+        // the last iteration's atom is not subtracted: no columns after it
+        const bool last = it + 1 == p->maxIter;
         if (prop) {
             if (graph) {
                 EMPI_HIP(hipGraphLaunch(p->prop_exec, s));
             } else if ((st = launch_proposal_iteration(p, s)) != EMPI_OK) {
                 return st;
             }
-        } else if (graph) {
+        } else if (graph && !last) {
             EMPI_HIP(hipGraphLaunch(p->iter_exec, s));
         } else {
             if ((st = launch_select(p, s)) != EMPI_OK) return st;
-            if (it + 1 < p->maxIter) {
+            if (!last) {
                 if ((st = launch_all_columns(p, s, 0)) != EMPI_OK) return st;
             }
         }
-        if ((it & 7) == 7) {
+        if (ring) {
+            const int slot = it % kPollRing;
+            hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, p->d_state.p, U, p->d_nactive.p);
+            EMPI_HIP(hipGetLastError());
+            EMPI_HIP(hipMemcpyAsync(p->h_nactive + slot, p->d_nactive.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            EMPI_HIP(hipEventRecord(p->poll_ev[slot], s));
+        } else if ((it & 7) == 7) {
             EMPI_HIP(hipMemcpyAsync(hs.data(), p->d_state.p, sizeof(UnitState) * U, hipMemcpyDeviceToHost, s));
             EMPI_HIP(hipStreamSynchronize(s));
             bool any = false;
