@@ -186,9 +186,17 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     const int per_xcd = gridDim.x / kXcds;
     const int chunk = (total + kXcds - 1) / kXcds;
     const int w_end = min(total, (xcd + 1) * chunk);
+    int lo = 0;   // work[lo] <= w for every w this workgroup visits from here on
     for (int w = xcd * chunk + static_cast<int>(blockIdx.x / kXcds); w < w_end; w += per_xcd) {
-        // item: last i with work[i] <= w
-        int lo = 0, hi = nitems - 1;
+        // item: last i with work[i] <= w.  w only grows, so gallop forward
+        // from the last item (a few loads when items are small and many)
+        // before the binary search
+        int step = 1;
+        while (lo + step < nitems && cd.work[lo + step] <= w) {
+            lo += step;
+            step <<= 1;
+        }
+        int hi = min(lo + step, nitems) - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (cd.work[mid] <= w) lo = mid; else hi = mid - 1;

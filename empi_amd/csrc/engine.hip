@@ -1636,7 +1636,10 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     }
 
     // region tables: time indices of each (scale, 256-sample block); column t
-    // sits at sample t (N-1)/(T-1) and belongs to block floor(of that / 256)
+    // sits at sample t (N-1)/(T-1) and belongs to block floor(of that / 256),
+    // with each block's first column rounded to the nearest boundary of the
+    // scale's column tiles (NCELL / G columns), so that evaluating a region
+    // never transforms a tile of which only part is evaluated
     for (size_t di = 0; di < p->dicts.size(); ++di) {
         const Dictionary& d = p->dicts[di];
         const long long N = d.N;
@@ -1645,11 +1648,17 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
         p->dict_B.push_back(B);
         for (const Scale& sc : d.scales) {
             const long long T = sc.T;
-            auto first_t = [&](long long b) { return (b * kBlock * (T - 1) + (N - 2)) / (N - 1); };
+            const int cpw = cells_per_wg(sc.Q);
+            const long long cols = cpw > sc.G ? cpw / sc.G : 1;   // powers of two
+            auto first_t = [&](long long b) {
+                const long long t = (b * kBlock * (T - 1) + (N - 2)) / (N - 1);
+                return (t + cols / 2) / cols * cols;
+            };
             for (int b = 0; b < B; ++b) {
                 RegionDev r;
                 r.tlo = static_cast<int>(std::min<long long>(first_t(b), T));
                 r.thi = static_cast<int>(std::min<long long>(first_t(b + 1) - 1, T - 1));
+                if (b == B - 1) r.thi = static_cast<int>(T - 1);
                 p->regtab_h.push_back(r);
             }
         }
