@@ -32,9 +32,22 @@ struct Lds {
     static constexpr int SIZE = NPTS + (NPTS >> pad_shift<Q>());
 };
 
+// The pass at sub-length Ns = 16 (classes whose first radix is 16) takes its
+// twiddles w^r = e^{-2 pi i k r / (16 R)}, k < 16, r < R, from a table
+// staged once per workgroup in LDS at [r][k]: a thread's k is the same in
+// every tile, the 16 values of k fill the 32 banks once (rows [k][r] would
+// put all sixteen on one bank), and R - 1 table reads replace one hardware
+// sine/cosine and R - 2 complex products
+template <int Q>
+constexpr int tw16_radix() { return Q > 16 && pick_radix(Q) == 16 ? pick_radix(Q / 16) : 0; }
+template <int Q>
+struct Tw16 {
+    static constexpr int SIZE = tw16_radix<Q>() > 0 ? 16 * tw16_radix<Q>() : 1;
+};
+
 // One Stockham autosort pass of radix R at sub-length Ns over all slots.
 template <int Q, int NT, int NPTS, int Ns, int R>
-__device__ inline void stockham_pass(float2* buf) {
+__device__ inline void stockham_pass(float2* buf, const float2* tw16) {
     constexpr int BPS = Q / R;
     constexpr int NB = NPTS / R;
     constexpr int PER = NB / NT;
@@ -54,7 +67,12 @@ __device__ inline void stockham_pass(float2* buf) {
         const int bi = tid + b * NT;
         const int slot = bi / BPS, j = bi % BPS;
         int k = j & (Ns - 1);
-        if constexpr (Ns > 1) {
+        if constexpr (Ns == 16) {
+            static_assert(R == tw16_radix<Q>(), "table radix");
+            const float2* col = tw16 + k;
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], col[16 * r]);
+        } else if constexpr (Ns > 1) {
             // twiddles are recomputed per pass rather than hoisted out of the
             // persistent loop: holding them for every pass costs more
             // registers than the hardware sine/cosine costs cycles
@@ -97,12 +115,12 @@ __device__ inline void stockham_pass(float2* buf) {
 }
 
 template <int Q, int NT, int NPTS, int Ns>
-__device__ inline void fft_passes(float2* buf) {
+__device__ inline void fft_passes(float2* buf, const float2* tw16) {
     if constexpr (Ns < Q) {
         constexpr int left = Q / Ns;
         constexpr int R = pick_radix(left);
-        stockham_pass<Q, NT, NPTS, Ns, R>(buf);
-        fft_passes<Q, NT, NPTS, Ns * R>(buf);
+        stockham_pass<Q, NT, NPTS, Ns, R>(buf, tw16);
+        fft_passes<Q, NT, NPTS, Ns * R>(buf, tw16);
     }
 }
 
@@ -170,9 +188,18 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     __shared__ float2 etl[kEtabLds + 1];    // ((1 - E)/K', 2E/K') of the item's scale; [kEtabLds] = (1/K', 0)
     // windows longer than 2Q samples (Q = 4096 only): e^{-2 pi i k2 / P} of each cell's two slots
     __shared__ float2 ftw[Q == kMaxSubTransform ? 2 * NCELL : 1];
+    __shared__ float2 tw16[Tw16<Q>::SIZE];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
+    if constexpr (tw16_radix<Q>() > 0) {
+        constexpr int R16 = tw16_radix<Q>();
+        for (int i = tid; i < 16 * R16; i += NT) {
+            double sn, cs;
+            sincospi(-2.0 * static_cast<double>((i / 16) * (i % 16)) / (16.0 * R16), &sn, &cs);
+            tw16[i] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
+        }
+    }
     const ClassDev cd = classes[cls];
     const int nitems = min(*cd.count, cd.max_items);
     const int total = cd.work[nitems];
@@ -375,7 +402,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 }
             }
             __syncthreads();
-            fft_passes<Q, NT, NPTS, 1>(buf);
+            fft_passes<Q, NT, NPTS, 1>(buf, tw16);
 
             // post: split the packed spectrum, phase-correct, product per bin.
             // element(rem, c, ci, i, run): bin `rem` (of 2Q) of cell c; i indexes
