@@ -677,9 +677,18 @@ __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev
 // n_A the atom's own norm.  racc takes the maximum over all bins f, racc_lo
 // over the low bins, which lie within f_L = (JB - 1) fs / Nfft of 0 and of
 // fs / 2: an atom far from both in frequency barely moves them.  The
-// frequency decay needs whole Gaussians: where the atom's support or a window
-// of the region is clipped by the signal's ends only the envelope bound
-// n_A n I PG(0) holds.  1% and 1e-9 cover rounding and the 3 s truncation.
+// frequency decay holds for the untruncated sum over every sample n.  The
+// map sums only n in [lo, hi] = [0, N-1] cut to the column's window (3 s)
+// and the atom's support (3 sA); the product of the two envelopes is
+// e^{-pi dt^2 / (s^2 + sA^2)} e^{-pi (n - c)^2 / sg_n^2}, a Gaussian of width
+// sg_n = sg fs samples about c = fs (t sA^2 + tA s^2) / (s^2 + sA^2), so the
+// samples cut off add at most I / sg_n times the tail sums
+//   sum_{k >= 0} e^{-pi (D + k)^2 / sg_n^2} <= e^{-pi D^2 / sg_n^2} / (1 - e^{-2 pi D / sg_n^2}),
+// D = c - lo + 1 below and hi + 1 - c above.  With lrint within 0.5 of its
+// argument both D are concave in t, so the region's end columns give their
+// minimum.  The envelope bound n_A n I PG(0) (every term in phase) caps the
+// result, and is all that remains when a cut reaches the Gaussian's centre.
+// 1% and 1e-9 cover rounding.
 __global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __restrict__ units,
                                                                const UnitState* __restrict__ state,
                                                                const ScaleDev* __restrict__ scales,
@@ -708,12 +717,26 @@ __global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __
         const double dt = st.tA < t0 ? t0 - st.tA : (st.tA > t1 ? st.tA - t1 : 0.0);
         const double v = s * s + sA * sA;
         const double overlap = s * sA / sqrt(v) * exp(-M_PI * dt * dt / v);
-        const bool clipped = lrint((t0 - kGaussHalfWidth * s) * ud.fs) < 0 ||
-                             lrint((t1 + kGaussHalfWidth * s) * ud.fs) > ud.N - 1 ||
-                             lrint((st.tA - kGaussHalfWidth * sA) * ud.fs) < 0 ||
-                             lrint((st.tA + kGaussHalfWidth * sA) * ud.fs) > ud.N - 1;
         const double sg2 = s * s * sA * sA / v;
         const double fs = ud.fs, fN = 0.5 * fs;
+        // tails cut off below lo and above hi, relative to I, at the region's end columns
+        const double sgn = sqrt(sg2) * fs, sgn2 = sgn * sgn;
+        // lrint(x) <= x + 0.5: lo from above, hi from below
+        const double loA = fmax(0.0, (st.tA - kGaussHalfWidth * sA) * fs + 0.5);
+        const double hiA = fmin(ud.N - 1.0, (st.tA + kGaussHalfWidth * sA) * fs - 0.5);
+        auto tail = [&](double D) {
+            return D > 0.0 ? exp(-M_PI * D * D / sgn2) / (1.0 - exp(-2.0 * M_PI * D / sgn2)) : HUGE_VAL;
+        };
+        double cut_lo = 0.0, cut_hi = 0.0;
+        for (int e = 0; e < 2; ++e) {
+            const double t = e ? t1 : t0;
+            const double c = fs * (t * sA * sA + st.tA * s * s) / v;
+            const double lo = fmax(loA, (t - kGaussHalfWidth * s) * fs + 0.5);
+            const double hi = fmin(hiA, (t + kGaussHalfWidth * s) * fs - 0.5);
+            cut_lo = fmax(cut_lo, tail(c - lo + 1.0));
+            cut_hi = fmax(cut_hi, tail(hi + 1.0 - c));
+        }
+        const double cut = (cut_lo + cut_hi) / sgn;
         // sum_m e^{-pi sg2 (x - m fs)^2}; sg fs >= sqrt 2, so |m| <= 3 suffices
         auto pg = [&](double x) {
             double acc = 0.0;
@@ -721,16 +744,13 @@ __global__ __launch_bounds__(kRegionThreads) void k_mark_dirty(const UnitDev* __
             return acc;
         };
         const double base = 1.01 * nA * sqrt(2.0 * M_SQRT2 / s) * overlap;
-        double rho_hi, rho_lo;
-        if (clipped) {
-            rho_hi = rho_lo = base * pg(0.0);
-        } else {
-            const double fL = max(sd.JB - 1, 0) * fs / sd.Nfft;
-            rho_hi = base * 0.5 * (pg(0.0) + pg(fmin(fA, fN - fA)));
-            const double d_diff = fmin(fmax(0.0, fA - fL), fmax(0.0, fN - fL - fA));
-            const double d_sum = fmax(0.0, fmin(fmin(fA, fs - fA - fL), fmin(fN - fA, fN - fL + fA)));
-            rho_lo = base * 0.5 * (pg(d_diff) + pg(d_sum));
-        }
+        const double env = base * pg(0.0);
+        const double fL = max(sd.JB - 1, 0) * fs / sd.Nfft;
+        const double d_diff = fmin(fmax(0.0, fA - fL), fmax(0.0, fN - fL - fA));
+        const double d_sum = fmax(0.0, fmin(fmin(fA, fs - fA - fL), fmin(fN - fA, fN - fL + fA)));
+        // fmin drops the NaN of 0 * HUGE_VAL (an atom of zero modulus)
+        const double rho_hi = fmin(env, base * (0.5 * (pg(0.0) + pg(fmin(fA, fN - fA))) + cut));
+        const double rho_lo = fmin(env, base * (0.5 * (pg(d_diff) + pg(d_sum)) + cut));
         R->racc += static_cast<float>(coef * fmin(1.0, rho_hi + 1e-9));
         R->racc_lo += static_cast<float>(coef * fmin(1.0, rho_lo + 1e-9));
     }

@@ -117,13 +117,16 @@ def _pg(sig2, fs, d):
     return sum(np.exp(-np.pi * sig2 * (d - m * fs) ** 2) for m in range(-3, 4))
 
 
-def overlap_rho(s, fs, Nfft, JB, t0, t1, params, clipped):
+def overlap_rho(s, fs, N, Nfft, JB, t0, t1, params):
     """(all bins, bins 1 <= j < JB) bounds on |u| of the unit atom `params`
-    at scale s, time in [t0, t1] (k_mark_dirty).  The discrete overlap of two
-    Gabor atoms is n_A n I / 2 |PG(fA - f) + PG(fA + f)| (Poisson summation,
-    I the envelope integral); the low bins lie within f_L of 0 and of fs/2;
-    where a window or the atom is clipped by the signal's ends only the
-    envelope bound n_A n I PG(0) holds."""
+    at scale s, time in [t0, t1] (k_mark_dirty).  The untruncated discrete
+    overlap of two Gabor atoms is n_A n I / 2 |PG(fA - f) + PG(fA + f)|
+    (Poisson summation, I the envelope integral); the low bins lie within f_L
+    of 0 and of fs/2.  The samples that the signal's ends, the window (3 s)
+    and the atom's support (3 sA) cut off add at most I / sg_n times the
+    Gaussian tail sums beyond the cuts (sg_n the width in samples of the two
+    envelopes' product, centred on c), taken at the region's end columns; the
+    envelope bound n_A n I PG(0) caps both."""
     modulus, amp, sA, fA, tA = params[0], params[1], params[3] / fs, params[4] * fs / 2, params[2] / fs
     nA = np.sqrt(fs) * abs(amp) / abs(modulus)
     dt = t0 - tA if tA < t0 else (tA - t1 if tA > t1 else 0.0)
@@ -132,15 +135,29 @@ def overlap_rho(s, fs, Nfft, JB, t0, t1, params, clipped):
     I = s * sA / np.sqrt(v) * np.exp(-np.pi * dt * dt / v)
     n = np.sqrt(2 * np.sqrt(2) / s)
     base = 1.01 * nA * n * I
-    if clipped:
-        r = min(1.0, base * _pg(sig2, fs, 0.0) + 1e-9)
-        return r, r
+    sgn2 = sig2 * fs * fs
+
+    def tail(D):
+        return np.exp(-np.pi * D * D / sgn2) / (1 - np.exp(-2 * np.pi * D / sgn2)) if D > 0 else np.inf
+
+    loA = max(0.0, (tA - 3 * sA) * fs + 0.5)
+    hiA = min(N - 1.0, (tA + 3 * sA) * fs - 0.5)
+    cut_lo = cut_hi = 0.0
+    for t in (t0, t1):
+        c = fs * (t * sA * sA + tA * s * s) / v
+        cut_lo = max(cut_lo, tail(c - max(loA, (t - 3 * s) * fs + 0.5) + 1))
+        cut_hi = max(cut_hi, tail(min(hiA, (t + 3 * s) * fs - 0.5) + 1 - c))
+    cut = (cut_lo + cut_hi) / np.sqrt(sgn2)
+    env = base * _pg(sig2, fs, 0.0)
     fN = fs / 2
     fL = max(JB - 1, 0) * fs / Nfft
-    hi = base * 0.5 * (_pg(sig2, fs, 0.0) + _pg(sig2, fs, min(fA, fN - fA)))
     d_diff = min(max(0.0, fA - fL), max(0.0, fN - fL - fA))
     d_sum = min(fA, fs - fA - fL, fN - fA, fN - fL + fA)
-    lo = base * 0.5 * (_pg(sig2, fs, d_diff) + _pg(sig2, fs, max(0.0, d_sum)))
+    if np.isfinite(cut):
+        hi = min(env, base * (0.5 * (_pg(sig2, fs, 0.0) + _pg(sig2, fs, min(fA, fN - fA))) + cut))
+        lo = min(env, base * (0.5 * (_pg(sig2, fs, d_diff) + _pg(sig2, fs, max(0.0, d_sum))) + cut))
+    else:
+        hi = lo = env
     return min(1.0, hi + 1e-9), min(1.0, lo + 1e-9)
 
 
@@ -150,14 +167,6 @@ def region_columns(d, k, b):
     T, N = d.scales[k].T, d.N
     first = lambda bb: (bb * 256 * (T - 1) + (N - 2)) // (N - 1)
     return min(first(b), T), min(first(b + 1) - 1, T - 1)
-
-
-def _clipped(d, k, tlo, thi, params):
-    s = d.scales[k].s
-    t0, t1 = tlo * d.tMax / (d.scales[k].T - 1), thi * d.tMax / (d.scales[k].T - 1)
-    sA, tA = params[3] / d.fs, params[2] / d.fs
-    return (np.rint((t0 - 3 * s) * d.fs) < 0 or np.rint((t1 + 3 * s) * d.fs) > d.N - 1 or
-            np.rint((tA - 3 * sA) * d.fs) < 0 or np.rint((tA + 3 * sA) * d.fs) > d.N - 1)
 
 
 def low_seeds(d, k, x):
@@ -186,7 +195,7 @@ def region_bounds(d, k, tlo, thi, P_old, seeds, atoms):
     rmax = float(np.max(P_old[:, tlo:thi + 1]))
     seed = float(np.max(seeds[tlo:thi + 1]))
     JB, gH, gL, lL = scale_factors(sc.s, d.fs, sc.Nfft)
-    rh = [overlap_rho(sc.s, d.fs, sc.Nfft, JB, t0, t1, p, _clipped(d, k, tlo, thi, p)) for p in atoms]
+    rh = [overlap_rho(sc.s, d.fs, d.N, sc.Nfft, JB, t0, t1, p) for p in atoms]
     racc_hi = sum(abs(p[0]) * r[0] for p, r in zip(atoms, rh))
     racc_lo = sum(abs(p[0]) * r[1] for p, r in zip(atoms, rh))
     old = (np.sqrt(rmax) + sum(abs(p[0]) * _old_rho(sc.s, d.fs, t0, t1, p) for p in atoms)) ** 2 * 1.0001
