@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--delta", type=float, default=1e-4)
     ap.add_argument("--top-k", type=int, default=8)
     ap.add_argument("--plain-runs", type=int, default=3, help="timed runs of the plain method (median)")
+    ap.add_argument("--unbatched-sample", type=int, default=32,
+                    help="signals of the one-plan-per-signal plain leg (0 skips it)")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--channels", type=int, default=64)
     ap.add_argument("--iterations", type=int, default=None)
@@ -186,6 +188,37 @@ def uneven_signals(args):
                 f"iterations")
 
 
+def unbatched_plain(xs, ptr, torch, tuning, sample):
+    """The plain method one signal at a time: one single-signal plan per
+    signal of a length-stratified sample (`sample` signals evenly spaced in
+    length order), run one after another with the same tuning table.  Plans
+    are created and warmed up before the timed loop."""
+    import empi_amd as E
+    offs = np.concatenate([[0], np.cumsum([len(x) for x in xs])])
+    order = np.argsort([len(x) for x in xs], kind="stable")
+    pick = [int(order[j]) for j in np.linspace(0, len(xs) - 1, sample).round().astype(int)]
+    plans = []
+    for i in pick:
+        p = E.Plan([E.ItemSpec(1, len(xs[i]), 256.0, E.SMP, 100, 99.0, 0.01)])
+        p.set_tuning(tuning)
+        plans.append((i, p, ptr + 4 * int(offs[i])))
+    for _, p, a in plans:
+        p.run(device_ptr=a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _, p, a in plans:
+        p.run(device_ptr=a)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    atoms = sum(int(p.result(0).counts.sum()) for _, p, _ in plans)
+    for _, p, _ in plans:
+        p.close()
+    progress(f"unbatched plain: {atoms} atoms of {len(pick)} signals in {sec:.2f} s")
+    return {"atoms_per_s": atoms / sec, "atoms": atoms, "seconds": sec, "signals": len(pick),
+            "sample": f"{len(pick)} signals evenly spaced in length order, one plan each, run in sequence",
+            "samples_total": int(sum(len(xs[i]) for i in pick))}
+
+
 def headline(args, torch):
     import empi_amd as E
     from empi_amd import proposal
@@ -211,6 +244,7 @@ def headline(args, torch):
     plain_res = [plain.result(i) for i in range(len(specs))]
     plain_entries = plain.entries_evaluated()
     plain.close()
+    unbatched = unbatched_plain(xs, ptr, torch, tuning, args.unbatched_sample) if args.unbatched_sample else None
 
     prop = E.Plan(specs)
     prop.set_tuning(tuning)
@@ -245,6 +279,10 @@ def headline(args, torch):
         },
         "roofline": roof, "kernels": kernels,
     }
+    if unbatched:
+        line["unbatched_plain"] = {**unbatched,
+                                   "batched_proposal_vs_unbatched": value / unbatched["atoms_per_s"],
+                                   "batched_plain_vs_unbatched": plain_rate / unbatched["atoms_per_s"]}
     # oracle samples: the two shortest signals whole, the median-length one over 20 iterations
     order = np.argsort([len(x) for x in xs])
     cases = []
