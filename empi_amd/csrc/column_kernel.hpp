@@ -15,8 +15,12 @@
 
 namespace empi {
 
-// E-table entries staged in LDS per work item (scales with more fall back to HBM)
-constexpr int kEtabLds = 128;
+// E-table entries staged in LDS per work item (scales with more fall back to
+// HBM).  Q <= 16 keeps 64: its scales have Nfft <= 128 at energyError 0.01,
+// so JT <= 33, and the 520 bytes saved keep the low-bin variant of Q = 16
+// within four workgroups per CU (40,960 bytes each)
+template <int Q>
+constexpr int etab_lds() { return Q <= 16 ? 64 : 128; }
 
 // LDS index with one float2 of padding per 8 points: the strided writes of
 // the radix-16/8 passes then spread over all banks.  Classes Q <= 16 run one
@@ -185,6 +189,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     __shared__ unsigned lowbest[LB ? NCELL : 1];   // FP32 bits of a non-negative maximum
     __shared__ unsigned long long tbest;
     __shared__ unsigned cnt_lds;
+    constexpr int kEtabLds = etab_lds<Q>();
     __shared__ float2 etl[kEtabLds + 1];    // ((1 - E)/K', 2E/K') of the item's scale; [kEtabLds] = (1/K', 0)
     // windows longer than 2Q samples (Q = 4096 only): e^{-2 pi i k2 / P} of each cell's two slots
     __shared__ float2 ftw[Q == kMaxSubTransform ? 2 * NCELL : 1];

@@ -2020,14 +2020,17 @@ empi_status plan_autotune(empi_plan* p, const float* signal, int on_device, int 
             for (int c = 0; c < kNumClasses && result == EMPI_OK; ++c) {
                 ClassJobs& cj = p->classes[mode][c];
                 if (cj.jobs.empty() || cj.tiles == 0) continue;
+                // with the proposal on, the single-channel classes run (and are
+                // timed as) their low-bin variants
+                const bool low = p->prop_mode != EMPI_PROPOSAL_OFF && mode == kModeSingle && p->d_celllow.p;
                 for (int v = 0; v < kTuneChoices && result == EMPI_OK; ++v) {
                     float best = INFINITY;
                     for (int r = 0; r < std::max(1, reps); ++r) {
                         if (hipEventRecord(e0, s) != hipSuccess) result = EMPI_DEVICE;
-                        column_launcher(mode, c, v)(column_grid(1 << (c + kMinLgQ), cj.tiles, v / kWaveVariants), s,
-                                                    p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p,
-                                                    p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p, nullptr,
-                                                    nullptr);
+                        column_launcher(mode, c, v, low)(column_grid(1 << (c + kMinLgQ), cj.tiles, v / kWaveVariants),
+                                                         s, p->d_classes.p, cj.slot, p->d_scales.p, p->d_units.p,
+                                                         p->d_state.p, p->d_res32.p, p->d_cells.p, p->d_tiles.p,
+                                                         nullptr, low ? p->d_celllow.p : nullptr);
                         float t = 0.0f;
                         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                             hipEventElapsedTime(&t, e0, e1) != hipSuccess)

@@ -144,17 +144,17 @@ def sweep(deltas, items=512, top_k=8, log=print):
     specs = [ItemSpec(1, len(x), 256.0, SMP, 100, 99.0, 0.01) for x in xs]
     dev = torch.from_numpy(np.concatenate([x.astype(np.float32) for x in xs])).to("cuda:0")
     net = load()
-    tuning = None
+    tuning = {}   # plain, proposal: each method's own autotuned table (the proposal runs low-bin kernels)
 
     def timed(delta):
-        nonlocal tuning
         plan = Plan(specs)
-        if tuning is None:
-            tuning = plan.autotune(device_ptr=dev.data_ptr(), repeats=2)
-        else:
-            plan.set_tuning(tuning)
+        key = delta is not None
         if delta is not None:
             plan.set_proposal(PROPOSAL_ON, net, top_k=top_k, delta=delta)
+        if key not in tuning:
+            tuning[key] = plan.autotune(device_ptr=dev.data_ptr(), repeats=2)
+        else:
+            plan.set_tuning(tuning[key])
         plan.run(device_ptr=dev.data_ptr())
         torch.cuda.synchronize()
         t0 = time.perf_counter()

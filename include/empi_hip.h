@@ -191,7 +191,9 @@ void        empi_plan_destroy(empi_plan* plan);
  * times every variant of every class on the full map build of this batch
  * (signal as for empi_plan_run; best of `repeats`, HIP events on the plan's
  * stream) and keeps the fastest; the table, empi_plan_tuning_size() ints
- * ([mode][class]: wave variant + 3 * grid variant), can be read and set. */
+ * ([mode][class]: wave variant + 3 * grid variant), can be read and set.
+ * On a plan with the proposal set, the single-channel classes are timed as
+ * the low-bin kernels the proposal runs. */
 int         empi_plan_tuning_size(void);
 empi_status empi_plan_autotune(empi_plan* plan, const float* signal, int signal_on_device, int repeats,
                                void* stream);
