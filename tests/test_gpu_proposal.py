@@ -130,5 +130,5 @@ def test_trained_network_gain():
     assert out["network"] < out["none"]
     assert out["network"] < out["plain"]
     # the stated margin over ranking by stale maximum: at least 0.5% fewer
-    # entries (1.1% measured: 0.664 against 0.671 of plain's entries)
+    # entries (2.3% measured: 0.540 against 0.553 of plain's entries)
     assert out["network"] <= 0.995 * out["stale"], (out["network"], out["stale"])
