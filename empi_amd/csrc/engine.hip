@@ -584,10 +584,9 @@ __global__ void k_stop_mc(const UnitDev* __restrict__ units, UnitState* __restri
 // (the candidates) and the top-K are evaluated exactly; then
 // every dirty region whose bound exceeds P* / (1 - delta), P* the best clean
 // product, is evaluated too (the exact fallback).  A region's bound is the
-// smaller of a Cauchy-Schwarz bound, csnorm * (residual energy under its
-// windows), which holds for every cell, and the stale maximum grown by the
-// accumulated atom-overlap bound (k_mark_dirty), which holds in the
-// continuous Gabor model.
+// smaller of a Cauchy-Schwarz bound, csnorm * (the largest residual energy
+// under one of its columns' windows), which holds for every cell, and the
+// stale maximum grown by the accumulated atom-overlap bound (k_mark_dirty).
 
 // affected time-index range of scale sd for the unit's last atom (widened by one)
 __device__ inline bool atom_range(const ScaleDev& sd, const UnitState& st, int* lo, int* hi) {
@@ -893,19 +892,48 @@ __global__ __launch_bounds__(kRegionThreads) void k_rows(
         const int l = r / ud.B, b = r % ud.B;
         const ScaleDev& sd = scales[ud.scale0 + l];
         const RegionDev rt = regtab[ud.regtab_off + r];
-        // union of the windows of columns tlo..thi
-        const double t0 = rt.tlo * sd.tMax / (sd.T - 1), t1 = rt.thi * sd.tMax / (sd.T - 1);
-        const double hw = kGaussHalfWidth * sd.s;
-        const long l0 = lrint((t0 - hw) * sd.fs), r1 = lrint((t1 + hw) * sd.fs);
-        const int iL = static_cast<int>(l0 > 0 ? l0 : 0), iR = static_cast<int>(r1 < N - 1 ? r1 : N - 1);
-        const float cs = sd.csnorm * static_cast<float>(pre[iR + 1] - pre[iL]);
         // bins j >= JB and bins 0, M: (B) at E <= 1e-4 or a real value;
         // bins 1 <= j < JB: the smaller of (A) from their |u| seeds and (B)
         const float hi = sqrtf(R->rmax) + R->racc * sd.gH;
         const float loA = (sqrtf(R->rlow) + R->racc_lo) * sd.gL;
         const float loB = sqrtf(R->rmax) + R->racc_lo * sd.lL;
         const float lo = fminf(loA, loB);
-        const float bound = fminf(cs, fmaxf(hi * hi, lo * lo) * 1.0001f);
+        const float grow = fmaxf(hi * hi, lo * lo) * 1.0001f;
+        // Cauchy-Schwarz per column: P(t, f) <= csnorm * (residual energy under
+        // column t's window [iL, iR]), at the region's largest.  A region of
+        // many narrow columns spans far more samples than any one window.
+        // Only needed where the growth bound fails (the certification's limit
+        // is never below this one).  Where the columns are many (more than 16
+        // and more than the starts below), windows of Wmax + S samples at
+        // starts S apart (S = Wmax / 2) cover every column's window
+        // [iL, iL + Wmax) between the first and the last start.
+        float bound = grow;
+        if (grow > limit || all_rows) {
+            const double hw = kGaussHalfWidth * sd.s;
+            auto win = [&](int t, int* iL, int* iR) {
+                const double tc = t * sd.tMax / (sd.T - 1);
+                const long wl = lrint((tc - hw) * sd.fs), wr = lrint((tc + hw) * sd.fs);
+                *iL = static_cast<int>(wl > 0 ? wl : 0);
+                *iR = static_cast<int>(wr < N - 1 ? wr : N - 1);
+            };
+            int il0, il1, ir0, ir1;
+            win(rt.tlo, &il0, &ir0);
+            win(rt.thi, &il1, &ir1);
+            const int S = max(1, sd.Wmax / 2);
+            double emax = 0.0;
+            const int ncol = rt.thi - rt.tlo + 1;
+            if (ncol > 16 && (il1 - il0) / S + 1 < ncol) {
+                for (int i = il0; i <= il1; i += S) emax = fmax(emax, pre[min(i + sd.Wmax + S, N)] - pre[i]);
+                emax = fmin(emax, pre[ir1 + 1] - pre[il0]);   // nor above the union of the windows
+            } else {
+                for (int t = rt.tlo; t <= rt.thi; ++t) {
+                    int iL, iR;
+                    win(t, &iL, &iR);
+                    emax = fmax(emax, pre[iR + 1] - pre[iL]);
+                }
+            }
+            bound = fminf(sd.csnorm * static_cast<float>(emax), grow);
+        }
         R->rbound = bound;
         const bool candidate = bound > limit;
         R->flags = candidate ? (R->flags | 4) : (R->flags & ~4);

@@ -292,3 +292,55 @@ def test_region_bound_holds_on_random_subtractions(seed):
             assert m <= new, (seed, fs, k, b, m, new)
             worst = max(worst, m / new)
     print("worst new max / bound", worst)
+
+
+# ------------------------------------------------------------ Cauchy-Schwarz cover
+#
+# k_rows bounds a dirty region's products by csnorm times the largest
+# residual energy under one of its columns' windows.  Where the columns are
+# many it takes windows of Wmax + S samples at starts S = Wmax / 2 apart
+# between the first and the last column's start instead, capped by the
+# energy under the union of the windows: restated here, that cover must never
+# fall below the exact per-column maximum (a gap in the starts would).
+
+def _cs_cover(d, k, tlo, thi, pre):
+    sc = d.scales[k]
+    N = d.N
+
+    def win(t):
+        tc = t * d.tMax / (sc.T - 1)
+        wl, wr = np.rint((tc - 3 * sc.s) * d.fs), np.rint((tc + 3 * sc.s) * d.fs)
+        return int(max(wl, 0)), int(min(wr, N - 1))
+
+    wmax = max(win(t)[1] - win(t)[0] + 1 for t in range(sc.T))
+    il0, il1 = win(tlo)[0], win(thi)[0]
+    S = max(1, wmax // 2)
+    ncol = thi - tlo + 1
+    if ncol > 16 and (il1 - il0) // S + 1 < ncol:
+        cover = max(pre[min(i + wmax + S, N)] - pre[i] for i in range(il0, il1 + 1, S))
+        return min(cover, pre[win(thi)[1] + 1] - pre[il0])
+    return max(pre[win(t)[1] + 1] - pre[win(t)[0]] for t in range(tlo, thi + 1))
+
+
+@pytest.mark.parametrize("fs,N", [(256.0, 1500), (128.0, 777)])
+def test_region_cs_cover(fs, N):
+    d = po.Dictionary(fs, N, 0.01)
+    rng = np.random.default_rng(int(N))
+    for trial in range(3):
+        x = synth.pink_noise(rng, N) * rng.uniform(0.5, 2.0, N) ** 3   # uneven energy
+        pre = np.concatenate([[0.0], np.cumsum(x * x)])
+        for k, sc in enumerate(d.scales):
+            for b in range((N + 255) // 256):
+                tlo, thi = region_columns(d, k, b)
+                if tlo > thi:
+                    continue
+                wins = []
+                for t in range(tlo, thi + 1):
+                    tc = t * d.tMax / (sc.T - 1)
+                    wins.append((int(max(np.rint((tc - 3 * sc.s) * fs), 0)),
+                                 int(min(np.rint((tc + 3 * sc.s) * fs), N - 1))))
+                exact = max(pre[r + 1] - pre[l] for l, r in wins)
+                union = pre[max(r for _, r in wins) + 1] - pre[min(l for l, _ in wins)]
+                cover = _cs_cover(d, k, tlo, thi, pre)
+                assert exact <= cover * (1 + 1e-12), (k, b, exact, cover)
+                assert cover <= union * (1 + 1e-12) + 1e-12, (k, b, cover, union)
