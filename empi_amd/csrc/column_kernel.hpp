@@ -212,33 +212,48 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
 
     // XCD-aware walk: workgroups are dealt round-robin over the 8 XCDs, so
     // workgroup b runs on XCD b % 8; each XCD takes one contiguous eighth of
-    // the work list, keeping neighbouring cells (overlapping windows) in one L2
+    // the work list, keeping neighbouring cells (overlapping windows) in one
+    // L2, and each of its workgroups a contiguous run of that eighth.
+    // Consecutive tiles then mostly belong to one work item: its item and
+    // job records stay in registers, so a tile's unit state, scale and unit
+    // records are loaded together instead of after a search and two
+    // dependent loads, and the E table is read once per scale.
     constexpr int kXcds = 8;
     const int xcd = blockIdx.x % kXcds;
     const int per_xcd = gridDim.x / kXcds;
     const int chunk = (total + kXcds - 1) / kXcds;
-    const int w_end = min(total, (xcd + 1) * chunk);
-    int lo = 0;   // work[lo] <= w for every w this workgroup visits from here on
-    for (int w = xcd * chunk + static_cast<int>(blockIdx.x / kXcds); w < w_end; w += per_xcd) {
-        // item: last i with work[i] <= w.  w only grows, so gallop forward
-        // from the last item (a few loads when items are small and many)
-        // before the binary search
-        int step = 1;
-        while (lo + step < nitems && cd.work[lo + step] <= w) {
-            lo += step;
-            step <<= 1;
+    const int x_lo = min(total, xcd * chunk), x_hi = min(total, (xcd + 1) * chunk);
+    const int run_len = (x_hi - x_lo + per_xcd - 1) / per_xcd;
+    const int w_beg = min(x_hi, x_lo + static_cast<int>(blockIdx.x / kXcds) * run_len);
+    const int w_end = min(x_hi, w_beg + run_len);
+    int lo = 0;             // work[lo] <= w for every w this workgroup visits from here on
+    int item_end = -1;      // first tile past the cached item
+    int item_w0 = 0, etl_scale = -1;
+    WorkItem item{};
+    Job job{};
+    for (int w = w_beg; w < w_end; ++w) {
+        if (w >= item_end) {
+            // item: last i with work[i] <= w.  w only grows, so gallop
+            // forward from the last item before the binary search
+            int step = 1;
+            while (lo + step < nitems && cd.work[lo + step] <= w) {
+                lo += step;
+                step <<= 1;
+            }
+            int hi = min(lo + step, nitems) - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (cd.work[mid] <= w) lo = mid; else hi = mid - 1;
+            }
+            item = cd.items[lo];
+            job = cd.jobs[item.job];
+            item_w0 = cd.work[lo];
+            item_end = cd.work[lo + 1];   // work has nitems + 1 entries, work[nitems] = total
         }
-        int hi = min(lo + step, nitems) - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (cd.work[mid] <= w) lo = mid; else hi = mid - 1;
-        }
-        const WorkItem item = cd.items[lo];
-        const Job job = cd.jobs[item.job];
-        const int tile = item.tile + (w - cd.work[lo]);
         const UnitState st = state[job.unit];
         const ScaleDev sd = scales[job.scale];
         const UnitDev ud = units[job.unit];
+        const int tile = item.tile + (w - item_w0);
         const int cell0 = tile * NCELL;
         const int ncells = sd.T * sd.G;
         const double s = sd.s, fs = sd.fs;
@@ -284,7 +299,10 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             cnt_lds = 0;
             tbest = 0ull;
         }
-        for (int i = tid; i <= kEtabLds; i += NT) etl[i] = i < sd.JT ? sd.etab[i] : make_float2(sd.invKp, 0.0f);
+        if (job.scale != etl_scale) {   // uniform: the previous tile's readers passed its last barrier
+            etl_scale = job.scale;
+            for (int i = tid; i <= kEtabLds; i += NT) etl[i] = i < sd.JT ? sd.etab[i] : make_float2(sd.invKp, 0.0f);
+        }
         int any = 0;
         for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
         if (!__syncthreads_or(any)) continue;
