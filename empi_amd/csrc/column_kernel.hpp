@@ -658,7 +658,9 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     accNyq += a * a;
                 }
             }
-            __syncthreads();
+            // the next channel reloads buf; a single channel's last reads of
+            // buf are ordered before the next tile by the barrier below
+            if constexpr (MULTI) __syncthreads();
         }
 
         // final products of multi-channel units and per-cell maximum
@@ -715,10 +717,9 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         if (mycnt) atomicAdd(&cnt_lds, mycnt);
         __syncthreads();
         // cell bests out, tile best over re-evaluated and untouched cells
-        unsigned long long tb = 0ull;
-        for (int c = tid; c < NCELL; c += NT) {
+        auto cell_out = [&](const int c) {
             const CellInfo& ci = info[c];
-            if (cell0 + c >= ncells) continue;
+            if (cell0 + c >= ncells) return 0ull;
             unsigned long long cb;
             if (ci.valid) {
                 cb = best[c];
@@ -727,16 +728,36 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             } else {
                 cb = cellbest[job.cell_off + cell0 + c];
             }
-            tb = max(tb, tile_key(cb, c, kTileCellBits));
+            return tile_key(cb, c, kTileCellBits);
+        };
+        if constexpr (NCELL <= 64) {
+            // the first wave holds every cell: it reduces the tile best
+            // itself, and it alone touches the cell records, the bests and
+            // the counters again before the next tile's first barrier (the
+            // other waves only restage the E table there, which this does
+            // not read), so no barrier is needed here
+            if (tid < 64) {
+                unsigned long long tb = tid < NCELL ? cell_out(tid) : 0ull;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) tb = max(tb, shfl_xor_u64(tb, off));
+                if (tid == 0) {
+                    tilebest[job.tile_off + tile] = tb;
+                    evaluated += cnt_lds;
+                    evaluated_ch += static_cast<unsigned long long>(cnt_lds) * (MULTI ? ud.C : 1);
+                }
+            }
+        } else {
+            unsigned long long tb = 0ull;
+            for (int c = tid; c < NCELL; c += NT) tb = max(tb, cell_out(c));
+            if (tb) atomicMax(&tbest, tb);
+            __syncthreads();
+            if (tid == 0) {
+                tilebest[job.tile_off + tile] = tbest;
+                evaluated += cnt_lds;
+                evaluated_ch += static_cast<unsigned long long>(cnt_lds) * (MULTI ? ud.C : 1);
+            }
+            __syncthreads();
         }
-        if (tb) atomicMax(&tbest, tb);
-        __syncthreads();
-        if (tid == 0) {
-            tilebest[job.tile_off + tile] = tbest;
-            evaluated += cnt_lds;
-            evaluated_ch += static_cast<unsigned long long>(cnt_lds) * (MULTI ? ud.C : 1);
-        }
-        __syncthreads();
     }
     if (tid == 0 && counter && evaluated) {
         constexpr int cls_ix = ilog2c(Q) - 1;

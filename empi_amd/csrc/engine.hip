@@ -923,9 +923,11 @@ __global__ __launch_bounds__(kRegionThreads) void k_rows(
             double emax = 0.0;
             const int ncol = rt.thi - rt.tlo + 1;
             if (ncol > 16 && (il1 - il0) / S + 1 < ncol) {
+#pragma unroll 4
                 for (int i = il0; i <= il1; i += S) emax = fmax(emax, pre[min(i + sd.Wmax + S, N)] - pre[i]);
                 emax = fmin(emax, pre[ir1 + 1] - pre[il0]);   // nor above the union of the windows
             } else {
+#pragma unroll 4
                 for (int t = rt.tlo; t <= rt.thi; ++t) {
                     int iL, iR;
                     win(t, &iL, &iR);
