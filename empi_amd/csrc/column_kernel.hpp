@@ -49,26 +49,41 @@ struct Tw16 {
     static constexpr int SIZE = tw16_radix<Q>() > 0 ? 16 * tw16_radix<Q>() : 1;
 };
 
+// Orders one wave's LDS accesses across its lanes.  LDS serves a wave's
+// instructions in issue order, so only the compiler must be kept from moving
+// accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One Stockham autosort pass of radix R at sub-length Ns over all slots.
-template <int Q, int NT, int NPTS, int Ns, int R>
+// WL (wave-local): wave w takes butterflies w NB/4 .. (w+1) NB/4 - 1, i.e.
+// slots w NSL/4 .. (w+1) NSL/4 - 1 in every pass, so its reads and writes
+// stay within its own slots and need no workgroup barrier.
+template <int Q, int NT, int NPTS, int Ns, int R, bool WL>
 __device__ inline void stockham_pass(float2* buf, const float2* tw16) {
     constexpr int BPS = Q / R;
     constexpr int NB = NPTS / R;
     constexpr int PER = NB / NT;
     static_assert(PER * NT == NB, "pass does not tile the workgroup");
+    static_assert(!WL || (NT == 256 && (NB / 4) % BPS == 0), "wave-local passes: four waves, whole slots each");
     const int tid = threadIdx.x;
+    const int b0 = WL ? (tid & ~63) * PER + (tid & 63) : tid;
+    constexpr int BSTEP = WL ? 64 : NT;
     float2 v[PER][R];
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
-        const int bi = tid + b * NT;
+        const int bi = b0 + b * BSTEP;
         const int slot = bi / BPS, j = bi % BPS;
 #pragma unroll
         for (int r = 0; r < R; ++r) v[b][r] = buf[lpad<Q>(slot * Q + j + r * BPS)];
     }
-    __syncthreads();
+    if constexpr (WL) wave_sync(); else __syncthreads();
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
-        const int bi = tid + b * NT;
+        const int bi = b0 + b * BSTEP;
         const int slot = bi / BPS, j = bi % BPS;
         int k = j & (Ns - 1);
         if constexpr (Ns == 16) {
@@ -115,16 +130,16 @@ __device__ inline void stockham_pass(float2* buf, const float2* tw16) {
             for (int r = 0; r < R; ++r) buf[lpad<Q>(base + r * Ns)] = v[b][r];
         }
     }
-    __syncthreads();
+    if constexpr (WL) wave_sync(); else __syncthreads();
 }
 
-template <int Q, int NT, int NPTS, int Ns>
+template <int Q, int NT, int NPTS, int Ns, bool WL>
 __device__ inline void fft_passes(float2* buf, const float2* tw16) {
     if constexpr (Ns < Q) {
         constexpr int left = Q / Ns;
         constexpr int R = pick_radix(left);
-        stockham_pass<Q, NT, NPTS, Ns, R>(buf, tw16);
-        fft_passes<Q, NT, NPTS, Ns * R>(buf, tw16);
+        stockham_pass<Q, NT, NPTS, Ns, R, WL>(buf, tw16);
+        fft_passes<Q, NT, NPTS, Ns * R, WL>(buf, tw16);
     }
 }
 
@@ -183,6 +198,16 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     constexpr int GRP = 2 * Q >= NT ? 2 * Q / NT : 1;      // consecutive elements sharing a cell
     static_assert(EPT % GRP == 0, "element groups");
     static_assert(NCELL <= (1 << kTileCellBits), "cell index of a tile-best key");
+    // Wave-local tiles (single channel, Q = 8..256, four waves): wave w owns
+    // cells w NCW .. (w+1) NCW - 1 and their slots from set-up through load,
+    // passes and products to the cell bests, so within a tile it orders its
+    // LDS accesses by itself.  The one workgroup barrier per tile is the
+    // tile-start vote; each wave leaves the maximum over its cells in tbw,
+    // and thread 0 writes the tile best after the next tile's vote, from the
+    // half of tbw that tile did not use.
+    constexpr bool WL = !MULTI && Q >= 8 && Q <= 256;
+    constexpr int NCW = WL ? NCELL / 4 : NCELL;
+    static_assert(!WL || (NT == 256 && NCW >= 1 && NCW <= 64), "wave-local tiles");
     __shared__ float2 buf[Lds<Q, NPTS>::SIZE];
     __shared__ CellInfo info[NCELL];
     __shared__ unsigned long long best[NCELL];
@@ -194,6 +219,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     // windows longer than 2Q samples (Q = 4096 only): e^{-2 pi i k2 / P} of each cell's two slots
     __shared__ float2 ftw[Q == kMaxSubTransform ? 2 * NCELL : 1];
     __shared__ float2 tw16[Tw16<Q>::SIZE];
+    __shared__ unsigned long long tbw[WL ? 8 : 1];   // per-wave tile maxima, two halves
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -231,6 +257,11 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     int item_w0 = 0, etl_scale = -1;
     WorkItem item{};
     Job job{};
+    int pend = -1, pend_half = 0, tb_half = 0;   // WL: tile-best slot awaiting its write, its half of tbw
+    auto wl_tile_best = [&]() {
+        const unsigned long long* tw = tbw + 4 * pend_half;
+        return max(max(tw[0], tw[1]), max(tw[2], tw[3]));
+    };
     for (int w = w_beg; w < w_end; ++w) {
         if (w >= item_end) {
             // item: last i with work[i] <= w.  w only grows, so gallop
@@ -259,9 +290,13 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         const double s = sd.s, fs = sd.fs;
         const int P = sd.P, G = sd.G, M = sd.M, F = sd.F, Nfft = sd.Nfft;
 
-        // bins this thread's cells contribute to the evaluation count
+        // bins this thread's cells contribute to the evaluation count.
+        // WL: lane l < NCW of wave w sets up cell w NCW + l
         unsigned mycnt = 0;
-        for (int c = tid; c < NCELL; c += NT) {
+        int any = 0;
+        const int c_own = WL ? (tid >> 6) * NCW + lane : tid;
+        const int c_end = WL ? (lane < NCW ? c_own + 1 : 0) : NCELL;
+        for (int c = c_own; c < c_end; c += NT) {
             CellInfo ci{};
             const int cell = cell0 + c;
             const bool exists = cell < ncells;
@@ -293,19 +328,29 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             info[c] = ci;
             best[c] = 0ull;
             if constexpr (LB) lowbest[c] = 0u;
+            any |= ci.valid;
             if (ci.valid) mycnt += cell_bins(ci, P, Q, F, M);
         }
-        if (tid == 0) {
+        if constexpr (WL) {
+            evaluated += mycnt;
+        } else if (tid == 0) {
             cnt_lds = 0;
             tbest = 0ull;
         }
-        if (job.scale != etl_scale) {   // uniform: the previous tile's readers passed its last barrier
+        if (job.scale != etl_scale) {
+            // uniform.  The previous tile's readers have passed its last
+            // barrier, except with wave-local tiles, whose waves may still be
+            // in their products
+            if constexpr (WL) __syncthreads();
             etl_scale = job.scale;
             for (int i = tid; i <= kEtabLds; i += NT) etl[i] = i < sd.JT ? sd.etab[i] : make_float2(sd.invKp, 0.0f);
         }
-        int any = 0;
-        for (int c = tid; c < NCELL; c += NT) any |= info[c].valid;
-        if (!__syncthreads_or(any)) continue;
+        const int any_all = __syncthreads_or(any);
+        if constexpr (WL) {
+            if (pend >= 0 && tid == 0) tilebest[pend] = wl_tile_best();
+            pend = -1;
+        }
+        if (!any_all) continue;
 
         const float invN = ldexpf(1.0f, -sd.lgNfft);   // Nfft is a power of two
         const float invM = 2.0f * invN;
@@ -346,18 +391,24 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             // 0 and P/2, and for GL = 1 a thread's pair index m = tid mod Q
             // is the same in all its cells, so the slot-b pre-twiddle is
             // computed once per tile
+            // WL: wave w loads pairs 512 w .. 512 w + 511 (its own cells), 64
+            // consecutive ones per step; a thread's m is then the same in all
+            // its cells only for Q <= 64
             const bool g1 = GL == 1 && __builtin_amdgcn_readfirstlane(G) == 1;
+            const int e0 = WL ? (tid & ~63) * (EPT / 2) + lane : tid;
+            constexpr int ESTEP = WL ? 64 : GL * NT;
+            constexpr bool TW1_ONCE = !WL || Q <= 64;
             float2 tw1 = make_float2(1.f, 0.f);
-            if (g1)
+            if (g1 && TW1_ONCE)
                 tw1 = cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(P / 2), static_cast<unsigned>(tid % Q))) *
                               invM);
 #pragma unroll
             for (int gi = 0; gi < EPT / 2 / GL; ++gi) {
-                const int c = (tid + gi * GL * NT) / Q;
+                const int c = (e0 + gi * ESTEP) / Q;
                 const CellInfo ci = info[c];
                 const int lim = ci.valid ? ci.W : 0;
                 const float cg = -sg * ci.dfrac;
-                const int m0 = (tid + gi * GL * NT) % Q;
+                const int m0 = (e0 + gi * ESTEP) % Q;
                 // pre-twiddles w_M^{k2 m} from exact revolution counts (k2 m < M <
                 // 2^24) at the group's first pair; the GL pairs of a group are NT
                 // apart in one cell, so later ones advance by w_M^{k2 NT}
@@ -367,6 +418,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 if (!g1) {
                     twa = cis_rev(-static_cast<float>(__umul24(uka, um0)) * invM);
                     twb = cis_rev(-static_cast<float>(__umul24(ukb, um0)) * invM);
+                } else if constexpr (!TW1_ONCE) {
+                    twb = cis_rev(-static_cast<float>(__umul24(static_cast<unsigned>(P / 2), um0)) * invM);
                 }
                 float2 sta = make_float2(1.f, 0.f), stb = make_float2(1.f, 0.f);
                 if constexpr (GL > 1) {
@@ -424,8 +477,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     }
                 }
             }
-            __syncthreads();
-            fft_passes<Q, NT, NPTS, 1>(buf, tw16);
+            if constexpr (WL) wave_sync(); else __syncthreads();
+            fft_passes<Q, NT, NPTS, 1, WL>(buf, tw16);
 
             // post: split the packed spectrum, phase-correct, product per bin.
             // element(rem, c, ci, i, run): bin `rem` (of 2Q) of cell c; i indexes
@@ -650,10 +703,10 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 }
             }
             // Nyquist bin k = M of g = 0 cells: X[M] = Re Z0 - Im Z0, u real, E = 1
-            if (tid < NCELL && F > M) {
-                const CellInfo& ci = info[tid];
+            if (c_own < c_end && F > M) {
+                const CellInfo& ci = info[c_own];
                 if (ci.valid && ci.ka == 0) {
-                    const float2 Z0 = buf[lpad<Q>((2 * tid) * Q)];
+                    const float2 Z0 = buf[lpad<Q>((2 * c_own) * Q)];
                     const float a = Z0.x - Z0.y;
                     accNyq += a * a;
                 }
@@ -707,15 +760,17 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 if ((lane & (SEG - 1)) == 0 && run) atomicMax(&best[c], run);
             }
         }
-        if (tid < NCELL && F > M) {
-            const CellInfo& ci = info[tid];
+        if (c_own < c_end && F > M) {
+            const CellInfo& ci = info[c_own];
             if (ci.valid && ci.ka == 0) {
                 // E = 1 at Nyquist: |u|^4 / (2 E a^2 / K') = K' a^2 / 2 summed over channels
-                atomicMax(&best[tid], pack_best(0.5f * sd.Kp * accNyq, static_cast<unsigned>(M)));
+                atomicMax(&best[c_own], pack_best(0.5f * sd.Kp * accNyq, static_cast<unsigned>(M)));
             }
         }
-        if (mycnt) atomicAdd(&cnt_lds, mycnt);
-        __syncthreads();
+        if constexpr (!WL) {
+            if (mycnt) atomicAdd(&cnt_lds, mycnt);
+            __syncthreads();
+        }
         // cell bests out, tile best over re-evaluated and untouched cells
         auto cell_out = [&](const int c) {
             const CellInfo& ci = info[c];
@@ -730,7 +785,16 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             }
             return tile_key(cb, c, kTileCellBits);
         };
-        if constexpr (NCELL <= 64) {
+        if constexpr (WL) {
+            wave_sync();
+            unsigned long long tb = c_own < c_end ? cell_out(c_own) : 0ull;
+#pragma unroll
+            for (int off = NCW / 2; off >= 1; off >>= 1) tb = max(tb, shfl_xor_u64(tb, off));
+            if (lane == 0) tbw[4 * tb_half + (tid >> 6)] = tb;
+            pend = job.tile_off + tile;
+            pend_half = tb_half;
+            tb_half ^= 1;
+        } else if constexpr (NCELL <= 64) {
             // the first wave holds every cell: it reduces the tile best
             // itself, and it alone touches the cell records, the bests and
             // the counters again before the next tile's first barrier (the
@@ -759,8 +823,17 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             __syncthreads();
         }
     }
-    if (tid == 0 && counter && evaluated) {
-        constexpr int cls_ix = ilog2c(Q) - 1;
+    constexpr int cls_ix = ilog2c(Q) - 1;
+    if constexpr (WL) {
+        __syncthreads();   // every wave has left its part of the last tile's best
+        if (pend >= 0 && tid == 0) tilebest[pend] = wl_tile_best();
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) evaluated += shfl_xor_u64(evaluated, off);
+        if (lane == 0 && counter && evaluated) {
+            atomicAdd(&counter[2 * cls_ix], evaluated);
+            atomicAdd(&counter[2 * cls_ix + 1], evaluated);
+        }
+    } else if (tid == 0 && counter && evaluated) {
         atomicAdd(&counter[2 * cls_ix], evaluated);
         atomicAdd(&counter[2 * cls_ix + 1], evaluated_ch);
     }
