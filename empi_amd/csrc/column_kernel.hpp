@@ -58,17 +58,18 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One Stockham autosort pass of radix R at sub-length Ns over all slots.
-// WL (wave-local): wave w takes butterflies w NB/4 .. (w+1) NB/4 - 1, i.e.
-// slots w NSL/4 .. (w+1) NSL/4 - 1 in every pass, so its reads and writes
-// stay within its own slots and need no workgroup barrier.
+// One Stockham autosort pass of radix R at sub-length Ns over all slots of
+// Q points.  WL (wave-local): of NW = NT / 64 waves, wave w takes butterflies
+// w NB/NW .. (w+1) NB/NW - 1, i.e. slots w NSL/NW .. (w+1) NSL/NW - 1 in
+// every pass, so its reads and writes stay within its own slots and need no
+// workgroup barrier.
 template <int Q, int NT, int NPTS, int Ns, int R, bool WL>
 __device__ inline void stockham_pass(float2* buf, const float2* tw16) {
     constexpr int BPS = Q / R;
     constexpr int NB = NPTS / R;
     constexpr int PER = NB / NT;
     static_assert(PER * NT == NB, "pass does not tile the workgroup");
-    static_assert(!WL || (NT == 256 && (NB / 4) % BPS == 0), "wave-local passes: four waves, whole slots each");
+    static_assert(!WL || (NB / (NT / 64)) % BPS == 0, "wave-local passes: whole slots per wave");
     const int tid = threadIdx.x;
     const int b0 = WL ? (tid & ~63) * PER + (tid & 63) : tid;
     constexpr int BSTEP = WL ? 64 : NT;
@@ -208,6 +209,22 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     constexpr bool WL = !MULTI && Q >= 8 && Q <= 256;
     constexpr int NCW = WL ? NCELL / 4 : NCELL;
     static_assert(!WL || (NT == 256 && NCW >= 1 && NCW <= 64), "wave-local tiles");
+    // Wave-local passes (Q = 8..4096, all modes): the Stockham passes run on
+    // slots of QP points that each wave holds whole.  Q <= 1024 passes over
+    // the Q-point slots themselves.  Q = 2048 and 4096 take the first step of
+    // a decimation in frequency at load time, in registers: with S = Q / 1024,
+    // the S inputs j + 1024 r (r < S) of a slot, which one thread loads, give
+    // y_s[j] = w_Q^{js} sum_r x[j + 1024 r] e^{-2 pi i rs/S}, and the slot's
+    // bin S k' + s is bin k' of the 1024-point transform of y_s.  A workgroup
+    // barrier then remains only after the load (where the tile is not
+    // wave-local) and before the products.  Q = 8192 (multichannel, 128
+    // registers per thread at 1024 threads) keeps workgroup passes: holding
+    // its sixteen decimation inputs spilled and ran 19% slower.
+    constexpr int QP = Q >= 2048 && Q <= 4096 ? 1024 : Q;
+    constexpr int S = Q / QP;
+    constexpr bool WLP = Q >= 8 && Q <= 4096;
+    static_assert(!WLP || (NPTS / (NT / 64)) % QP == 0, "wave-local passes");
+    static_assert(S == 1 || Q / NT == 8, "decimation step: eight pairs per thread, S of them 1024 apart");
     __shared__ float2 buf[Lds<Q, NPTS>::SIZE];
     __shared__ CellInfo info[NCELL];
     __shared__ unsigned long long best[NCELL];
@@ -218,13 +235,13 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     __shared__ float2 etl[kEtabLds + 1];    // ((1 - E)/K', 2E/K') of the item's scale; [kEtabLds] = (1/K', 0)
     // windows longer than 2Q samples (Q = 4096 only): e^{-2 pi i k2 / P} of each cell's two slots
     __shared__ float2 ftw[Q == kMaxSubTransform ? 2 * NCELL : 1];
-    __shared__ float2 tw16[Tw16<Q>::SIZE];
+    __shared__ float2 tw16[Tw16<QP>::SIZE];
     __shared__ unsigned long long tbw[WL ? 8 : 1];   // per-wave tile maxima, two halves
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    if constexpr (tw16_radix<Q>() > 0) {
-        constexpr int R16 = tw16_radix<Q>();
+    if constexpr (tw16_radix<QP>() > 0) {
+        constexpr int R16 = tw16_radix<QP>();
         for (int i = tid; i < 16 * R16; i += NT) {
             double sn, cs;
             sincospi(-2.0 * static_cast<double>((i / 16) * (i % 16)) / (16.0 * R16), &sn, &cs);
@@ -426,10 +443,9 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     sta = cis_rev(-static_cast<float>(__umul24(uka, static_cast<unsigned>(NT))) * invM);
                     stb = cis_rev(-static_cast<float>(__umul24(ukb, static_cast<unsigned>(NT))) * invM);
                 }
-#pragma unroll 2
-                for (int q = 0; q < GL; ++q) {
-                    const int m = m0 + q * NT;   // GL > 1: one cell, no wrap
-                    float2 za = make_float2(0.f, 0.f), zb = make_float2(0.f, 0.f);
+                // slot a and b inputs of pair m, pre-twiddled by ta and tb
+                auto load_pair = [&](const int m, const float2 ta_m, const float2 tb_m, float2& za, float2& zb) {
+                    za = zb = make_float2(0.f, 0.f);
                     // pairs past the window (and every later section) are zero
                     if (2 * m < lim) {
                         float2 z = window_pair(ci, lim, cg, m);
@@ -462,23 +478,76 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                                 zin_b = sb;
                             }
                         }
-                        za = ci.ka == 0 ? z : cmul(z, twa);
-                        if (ci.kb >= 0) zb = cmul(zin_b, twb);
+                        za = ci.ka == 0 ? z : cmul(z, ta_m);
+                        if (ci.kb >= 0) zb = cmul(zin_b, tb_m);
                     }
-                    const int pa = lpad<Q>((2 * c) * Q + m);
-                    buf[pa] = za;
-                    if constexpr (Q % (1 << pad_shift<Q>()) == 0)
-                        buf[pa + Q + (Q >> pad_shift<Q>())] = zb;   // lpad(i + Q) = lpad(i) + lpad(Q)
-                    else
-                        buf[lpad<Q>((2 * c + 1) * Q + m)] = zb;
-                    if constexpr (GL > 1) {
+                };
+                if constexpr (S == 1) {
+#pragma unroll 2
+                    for (int q = 0; q < GL; ++q) {
+                        const int m = m0 + q * NT;   // GL > 1: one cell, no wrap
+                        float2 za, zb;
+                        load_pair(m, twa, twb, za, zb);
+                        const int pa = lpad<Q>((2 * c) * Q + m);
+                        buf[pa] = za;
+                        if constexpr (Q % (1 << pad_shift<Q>()) == 0)
+                            buf[pa + Q + (Q >> pad_shift<Q>())] = zb;   // lpad(i + Q) = lpad(i) + lpad(Q)
+                        else
+                            buf[lpad<Q>((2 * c + 1) * Q + m)] = zb;
+                        if constexpr (GL > 1) {
+                            if (ci.ka != 0) twa = cmul(twa, sta);
+                            twb = cmul(twb, stb);
+                        }
+                    }
+                } else {
+                    // the thread's pairs j + 1024 r, r < S, for its GL / S values
+                    // of j (NT apart); pre-twiddles advance by w_M^{1024 k2} along r
+                    constexpr int GS = GL / S;
+                    const float2 sra = cis_rev(-static_cast<float>(__umul24(uka, static_cast<unsigned>(QP))) * invM);
+                    const float2 srb = cis_rev(-static_cast<float>(__umul24(ukb, static_cast<unsigned>(QP))) * invM);
+#pragma unroll 1
+                    for (int qq = 0; qq < GS; ++qq) {
+                        const int j = m0 + qq * NT;
+                        float2 va[S], vb[S];
+                        float2 ta = twa, tb = twb;
+#pragma unroll
+                        for (int r = 0; r < S; ++r) {
+                            load_pair(j + r * QP, ta, tb, va[r], vb[r]);
+                            if (ci.ka != 0) ta = cmul(ta, sra);
+                            tb = cmul(tb, srb);
+                        }
+                        dft<S>(va);
+                        dft<S>(vb);
+                        // w_Q^{js}: j / Q is an exact revolution count
+                        const float2 d1 = cis_rev(-static_cast<float>(j) * (1.0f / static_cast<float>(Q)));
+                        float2 ds = d1;
+#pragma unroll
+                        for (int r = 1; r < S; ++r) {
+                            va[r] = cmul(va[r], ds);
+                            vb[r] = cmul(vb[r], ds);
+                            if (r + 1 < S) ds = cmul(ds, d1);
+                        }
+                        // y_s of slot a at sub-transform 2c S + s, of slot b at (2c + 1) S + s
+                        const int pa = lpad<Q>((2 * c) * Q + j);
+#pragma unroll
+                        for (int r = 0; r < S; ++r) {
+                            buf[pa + r * (QP + (QP >> pad_shift<Q>()))] = va[r];
+                            buf[pa + (S + r) * (QP + (QP >> pad_shift<Q>()))] = vb[r];
+                        }
                         if (ci.ka != 0) twa = cmul(twa, sta);
                         twb = cmul(twb, stb);
                     }
                 }
             }
             if constexpr (WL) wave_sync(); else __syncthreads();
-            fft_passes<Q, NT, NPTS, 1, WL>(buf, tw16);
+            fft_passes<QP, NT, NPTS, 1, WLP>(buf, tw16);
+            if constexpr (WLP && !WL) __syncthreads();   // the products read both slots of a cell
+
+            // LDS position of bin k1 of a slot's transform (sub-transform k1 mod S, bin k1 / S of it)
+            auto xat = [](const int sl, const int k1) {
+                if constexpr (S == 1) return lpad<Q>(sl * Q + k1);
+                else return lpad<Q>((sl * S + (k1 & (S - 1))) * QP + (k1 >> ilog2c(S)));
+            };
 
             // post: split the packed spectrum, phase-correct, product per bin.
             // element(rem, c, ci, i, run): bin `rem` (of 2Q) of cell c; i indexes
@@ -545,10 +614,10 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(k1))) + k2;
                 if (!(ci.valid && k2 >= 0 && k < F)) return;
                 const int slot = 2 * c + half;
-                const float2 Zk = buf[lpad<Q>(slot * Q + k1)];
+                const float2 Zk = buf[xat(slot, k1)];
                 float2 Zp;
-                if (k2 == 0) Zp = buf[lpad<Q>(slot * Q + ((Q - k1) & (Q - 1)))];
-                else Zp = buf[lpad<Q>(((2 * k2 == P) ? slot : (slot ^ 1)) * Q + (Q - 1 - k1))];
+                if (k2 == 0) Zp = buf[xat(slot, (Q - k1) & (Q - 1))];
+                else Zp = buf[xat((2 * k2 == P) ? slot : (slot ^ 1), Q - 1 - k1)];
                 const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
                 const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
                 const float2 wv = cis_rev(-static_cast<float>(k) * invN);
@@ -579,8 +648,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 }
                 const int k = static_cast<int>(__umul24(static_cast<unsigned>(P), static_cast<unsigned>(ka1))) + k2;
                 const int km = M - k;
-                const float2 Zk = buf[lpad<Q>(sa * Q + ka1)];
-                const float2 Zp = buf[lpad<Q>(sb * Q + kb1)];
+                const float2 Zk = buf[xat(sa, ka1)];
+                const float2 Zp = buf[xat(sb, kb1)];
                 const float Ar = Zk.x + Zp.x, Ai = Zk.y - Zp.y;
                 const float Br = Zk.x - Zp.x, Bi = Zk.y + Zp.y;
                 if (direct) wv = cis_rev(-static_cast<float>(k) * invN);
@@ -630,7 +699,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                 }
                 if (ci.ka == 0 && u == 0 && M / 2 < F) {
                     // (a, Q/2): bin M/2, its own mirror (Z[k] = Z[M-k])
-                    const float2 Zh = buf[lpad<Q>(2 * c * Q + Q / 2)];
+                    const float2 Zh = buf[xat(2 * c, Q / 2)];
                     const float2 wh = cis_rev(-static_cast<float>(M / 2) * invN);
                     const float hBi = Zh.y + Zh.y;                 // B = (0, 2 Im Zh), A = (2 Re Zh, 0)
                     const float hpr = -wh.y * hBi, hpi = wh.x * hBi;
@@ -706,7 +775,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             if (c_own < c_end && F > M) {
                 const CellInfo& ci = info[c_own];
                 if (ci.valid && ci.ka == 0) {
-                    const float2 Z0 = buf[lpad<Q>((2 * c_own) * Q)];
+                    const float2 Z0 = buf[xat(2 * c_own, 0)];
                     const float a = Z0.x - Z0.y;
                     accNyq += a * a;
                 }
