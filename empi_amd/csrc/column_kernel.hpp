@@ -199,14 +199,14 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     constexpr int GRP = 2 * Q >= NT ? 2 * Q / NT : 1;      // consecutive elements sharing a cell
     static_assert(EPT % GRP == 0, "element groups");
     static_assert(NCELL <= (1 << kTileCellBits), "cell index of a tile-best key");
-    // Wave-local tiles (single channel, Q = 8..256, four waves): wave w owns
+    // Wave-local tiles (single channel, Q = 8..512, four waves): wave w owns
     // cells w NCW .. (w+1) NCW - 1 and their slots from set-up through load,
     // passes and products to the cell bests, so within a tile it orders its
     // LDS accesses by itself.  The one workgroup barrier per tile is the
     // tile-start vote; each wave leaves the maximum over its cells in tbw,
     // and thread 0 writes the tile best after the next tile's vote, from the
     // half of tbw that tile did not use.
-    constexpr bool WL = !MULTI && Q >= 8 && Q <= 256;
+    constexpr bool WL = !MULTI && Q >= 8 && Q <= 512;
     constexpr int NCW = WL ? NCELL / 4 : NCELL;
     static_assert(!WL || (NT == 256 && NCW >= 1 && NCW <= 64), "wave-local tiles");
     // Wave-local passes (Q = 8..4096, all modes): the Stockham passes run on
@@ -409,8 +409,9 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             // is the same in all its cells, so the slot-b pre-twiddle is
             // computed once per tile
             // WL: wave w loads pairs 512 w .. 512 w + 511 (its own cells), 64
-            // consecutive ones per step; a thread's m is then the same in all
-            // its cells only for Q <= 64
+            // consecutive ones per step (Q = 512: the cell's first half, each
+            // step's second pair NT further); a thread's m is then the same in
+            // all its cells only for Q <= 64
             const bool g1 = GL == 1 && __builtin_amdgcn_readfirstlane(G) == 1;
             const int e0 = WL ? (tid & ~63) * (EPT / 2) + lane : tid;
             constexpr int ESTEP = WL ? 64 : GL * NT;
@@ -706,7 +707,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                     from_x(M / 2, 0.5f * (Zh.x + Zh.x + hpi), -0.5f * hpr, ci, 0, run, low);
                 }
             };
-            constexpr bool CONTIG = !MULTI && Q >= 8 && Q <= 256;
+            constexpr bool CONTIG = !MULTI && Q >= 8 && Q <= 512;
             if constexpr (CONTIG) {
                 // Small sub-transforms: a thread's EPT / 2 mirror pairs are
                 // consecutive pairs of one cell, visited in a rotated order that
