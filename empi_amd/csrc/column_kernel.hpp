@@ -200,9 +200,12 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     static_assert(EPT % GRP == 0, "element groups");
     static_assert(NCELL <= (1 << kTileCellBits), "cell index of a tile-best key");
     // Wave-local tiles (single channel, Q = 8..512, four waves): wave w owns
-    // cells w NCW .. (w+1) NCW - 1 and their slots from set-up through load,
-    // passes and products to the cell bests, so within a tile it orders its
-    // LDS accesses by itself.  The one workgroup barrier per tile is the
+    // LDS cell positions w NCW .. (w+1) NCW - 1 and their slots from set-up
+    // through load, passes and products to the cell bests, so within a tile
+    // it orders its LDS accesses by itself.  Position w NCW + l holds the
+    // tile's cell 4 l + w: neighbouring cells, which an atom re-evaluates
+    // together, are dealt round-robin over the waves, and a wave none of
+    // whose cells is re-evaluated skips its load, passes and products.  The one workgroup barrier per tile is the
     // tile-start vote; each wave leaves the maximum over its cells in tbw,
     // and thread 0 writes the tile best after the next tile's vote, from the
     // half of tbw that tile did not use.
@@ -237,6 +240,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     __shared__ float2 ftw[Q == kMaxSubTransform ? 2 * NCELL : 1];
     __shared__ float2 tw16[Tw16<QP>::SIZE];
     __shared__ unsigned long long tbw[WL ? 8 : 1];   // per-wave tile maxima, two halves
+    __shared__ unsigned long long evw[WL ? 4 : 1];   // per-wave entry counts at the end
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -308,14 +312,15 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         const int P = sd.P, G = sd.G, M = sd.M, F = sd.F, Nfft = sd.Nfft;
 
         // bins this thread's cells contribute to the evaluation count.
-        // WL: lane l < NCW of wave w sets up cell w NCW + l
+        // WL: lane l < NCW of wave w sets up position w NCW + l, cell 4 l + w
         unsigned mycnt = 0;
         int any = 0;
         const int c_own = WL ? (tid >> 6) * NCW + lane : tid;
         const int c_end = WL ? (lane < NCW ? c_own + 1 : 0) : NCELL;
+        const int lc_own = WL ? lane * 4 + (tid >> 6) : tid;   // the tile's cell at c_own
         for (int c = c_own; c < c_end; c += NT) {
             CellInfo ci{};
-            const int cell = cell0 + c;
+            const int cell = cell0 + (WL ? lc_own : c);
             const bool exists = cell < ncells;
             ci.valid = exists;
             ci.kb = -1;
@@ -373,7 +378,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
         const float invM = 2.0f * invN;
         const float Kq = 0.25f * sd.Kp;
         constexpr bool mmp1 = MODE == kModeMmp1;
-        const int C = MULTI ? ud.C : 1;
+        // WL: a wave none of whose cells is re-evaluated has no channel to load
+        const int C = MULTI ? ud.C : (!WL || __ballot(any) != 0ull ? 1 : 0);
 
         float acc0[NACC], acc1[NACC3], acc2[NACC3];
         float accNyq = 0.0f;
@@ -842,22 +848,22 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             __syncthreads();
         }
         // cell bests out, tile best over re-evaluated and untouched cells
-        auto cell_out = [&](const int c) {
+        auto cell_out = [&](const int c, const int lc) {   // LDS position c holds the tile's cell lc
             const CellInfo& ci = info[c];
-            if (cell0 + c >= ncells) return 0ull;
+            if (cell0 + lc >= ncells) return 0ull;
             unsigned long long cb;
             if (ci.valid) {
                 cb = best[c];
-                cellbest[job.cell_off + cell0 + c] = cb;
-                if constexpr (LB) celllow[job.cell_off + cell0 + c] = __uint_as_float(lowbest[c]);
+                cellbest[job.cell_off + cell0 + lc] = cb;
+                if constexpr (LB) celllow[job.cell_off + cell0 + lc] = __uint_as_float(lowbest[c]);
             } else {
-                cb = cellbest[job.cell_off + cell0 + c];
+                cb = cellbest[job.cell_off + cell0 + lc];
             }
-            return tile_key(cb, c, kTileCellBits);
+            return tile_key(cb, lc, kTileCellBits);
         };
         if constexpr (WL) {
             wave_sync();
-            unsigned long long tb = c_own < c_end ? cell_out(c_own) : 0ull;
+            unsigned long long tb = c_own < c_end ? cell_out(c_own, lc_own) : 0ull;
 #pragma unroll
             for (int off = NCW / 2; off >= 1; off >>= 1) tb = max(tb, shfl_xor_u64(tb, off));
             if (lane == 0) tbw[4 * tb_half + (tid >> 6)] = tb;
@@ -871,7 +877,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             // other waves only restage the E table there, which this does
             // not read), so no barrier is needed here
             if (tid < 64) {
-                unsigned long long tb = tid < NCELL ? cell_out(tid) : 0ull;
+                unsigned long long tb = tid < NCELL ? cell_out(tid, tid) : 0ull;
 #pragma unroll
                 for (int off = 32; off >= 1; off >>= 1) tb = max(tb, shfl_xor_u64(tb, off));
                 if (tid == 0) {
@@ -882,7 +888,7 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             }
         } else {
             unsigned long long tb = 0ull;
-            for (int c = tid; c < NCELL; c += NT) tb = max(tb, cell_out(c));
+            for (int c = tid; c < NCELL; c += NT) tb = max(tb, cell_out(c, c));
             if (tb) atomicMax(&tbest, tb);
             __syncthreads();
             if (tid == 0) {
@@ -893,19 +899,24 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
             __syncthreads();
         }
     }
-    constexpr int cls_ix = ilog2c(Q) - 1;
+    // one counter update per workgroup, into its XCD's copy
+    unsigned long long* const cnt = counter ? counter + 2 * kNumClasses * xcd + 2 * (ilog2c(Q) - 1) : nullptr;
     if constexpr (WL) {
-        __syncthreads();   // every wave has left its part of the last tile's best
-        if (pend >= 0 && tid == 0) tilebest[pend] = wl_tile_best();
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) evaluated += shfl_xor_u64(evaluated, off);
-        if (lane == 0 && counter && evaluated) {
-            atomicAdd(&counter[2 * cls_ix], evaluated);
-            atomicAdd(&counter[2 * cls_ix + 1], evaluated);
+        if (lane == 0) evw[tid >> 6] = evaluated;
+        __syncthreads();   // every wave has left its part of the last tile's best and its count
+        if (tid == 0) {
+            if (pend >= 0) tilebest[pend] = wl_tile_best();
+            const unsigned long long n = evw[0] + evw[1] + evw[2] + evw[3];
+            if (cnt && n) {
+                atomicAdd(&cnt[0], n);
+                atomicAdd(&cnt[1], n);
+            }
         }
-    } else if (tid == 0 && counter && evaluated) {
-        atomicAdd(&counter[2 * cls_ix], evaluated);
-        atomicAdd(&counter[2 * cls_ix + 1], evaluated_ch);
+    } else if (tid == 0 && cnt && evaluated) {
+        atomicAdd(&cnt[0], evaluated);
+        atomicAdd(&cnt[1], evaluated_ch);
     }
 }
 

@@ -15,6 +15,12 @@ constexpr int kMaxQ = kMaxSubTransformMulti;   // 8192: multichannel modes only
 constexpr int kMinLgQ = 1;    // Q = 2
 constexpr int kMaxLgQ = 13;   // Q = 8192
 constexpr int kNumClasses = kMaxLgQ - kMinLgQ + 1;
+// The column kernels count re-evaluated entries into one copy of the
+// per-class counters per XCD (workgroup b runs on XCD b mod 8), so that
+// same-address atomics from different XCDs do not queue behind each other;
+// the host sums the copies.
+constexpr int kCounterCopies = 8;
+constexpr int kCounterWords = 2 * kNumClasses * kCounterCopies;
 
 // Workgroup geometry of class Q: NSL slots of Q complex points in LDS (4096
 // points in at most 512 slots up to Q = 1024, two slots from Q = 2048 on),

@@ -1437,7 +1437,8 @@ struct empi_plan {
     std::vector<empi_atom> h_atoms;
     std::vector<double> h_energies;
     std::vector<UnitState> h_state;
-    unsigned long long h_counter[2 * kNumClasses] = {};
+    unsigned long long h_counter[2 * kNumClasses] = {};          // summed over the XCD copies
+    unsigned long long h_counter_copies[kCounterWords] = {};
     bool has_results = false;
 
     // one MP iteration (select, plan, every column class) captured once
@@ -1769,7 +1770,7 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
     if ((st = p->d_energies.alloc(static_cast<size_t>(energy_off))) != EMPI_OK) return st;
     if ((st = p->d_fitv.alloc(p->units.size() * static_cast<size_t>(max_c))) != EMPI_OK) return st;
     if ((st = p->d_scratch.alloc(static_cast<size_t>(scratch_off))) != EMPI_OK) return st;
-    if ((st = p->d_counter.alloc(2 * kNumClasses)) != EMPI_OK) return st;
+    if ((st = p->d_counter.alloc(kCounterWords)) != EMPI_OK) return st;
     for (size_t ui = 0; ui < p->units.size(); ++ui) {
         if (p->units[ui].C > 1) {
             p->mc_units.push_back(static_cast<int>(ui));
@@ -2127,7 +2128,7 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
         in = p->d_input.p;
     }
     const int U = static_cast<int>(p->units.size());
-    EMPI_HIP(hipMemsetAsync(p->d_counter.p, 0, sizeof(unsigned long long) * 2 * kNumClasses, s));
+    EMPI_HIP(hipMemsetAsync(p->d_counter.p, 0, sizeof(unsigned long long) * kCounterWords, s));
     EMPI_HIP(hipMemsetAsync(p->d_atoms.p, 0, sizeof(empi_atom) * p->d_atoms.n, s));
     for (int c = 0; c < kNumClasses; ++c) p->ev_used[c] = 0;
     {
@@ -2240,7 +2241,7 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     EMPI_HIP(hipMemcpyAsync(p->h_energies.data(), p->d_energies.p, sizeof(double) * p->d_energies.n,
                             hipMemcpyDeviceToHost, s));
     EMPI_HIP(hipMemcpyAsync(p->h_state.data(), p->d_state.p, sizeof(UnitState) * U, hipMemcpyDeviceToHost, s));
-    EMPI_HIP(hipMemcpyAsync(p->h_counter, p->d_counter.p, sizeof(unsigned long long) * 2 * kNumClasses,
+    EMPI_HIP(hipMemcpyAsync(p->h_counter_copies, p->d_counter.p, sizeof(unsigned long long) * kCounterWords,
                             hipMemcpyDeviceToHost, s));
     if (prop) {
         EMPI_HIP(hipMemcpyAsync(p->h_pstats, p->d_pstats.p, sizeof(unsigned long long) * 5, hipMemcpyDeviceToHost, s));
@@ -2248,6 +2249,10 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
         std::fill(p->h_pstats, p->h_pstats + 5, 0ull);
     }
     EMPI_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < 2 * kNumClasses; ++i) {
+        p->h_counter[i] = 0;
+        for (int x = 0; x < kCounterCopies; ++x) p->h_counter[i] += p->h_counter_copies[2 * kNumClasses * x + i];
+    }
     if (p->profiling) {
         empi_status pst = collect_profile(p);
         if (pst != EMPI_OK) return pst;
@@ -2728,7 +2733,7 @@ empi_status empi_ws_compute(empi_ws* w, const double* signal) {
     ex = std::max(-100, std::min(100, ex));
     us.xscale = std::ldexp(1.0f, -ex);
     EMPI_HIP(hipMemcpyAsync(p->d_state.p, &us, sizeof us, hipMemcpyHostToDevice, s));
-    EMPI_HIP(hipMemsetAsync(p->d_counter.p, 0, sizeof(unsigned long long) * 2 * kNumClasses, s));
+    EMPI_HIP(hipMemsetAsync(p->d_counter.p, 0, sizeof(unsigned long long) * kCounterWords, s));
     hipLaunchKernelGGL(k_mirror, dim3(1), dim3(kSelectThreads), 0, s, p->d_units.p, p->d_state.p, p->d_res.p,
                        p->d_res32.p);
     empi_status st = launch_all_columns(p, s, 1);
