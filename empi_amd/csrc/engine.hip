@@ -62,33 +62,43 @@ constexpr int kSelectThreads = 256;
 // every column for the full build.
 constexpr int kPlanThreads = 1024;
 
-// exclusive prefix of the items' tile counts; work[count] = total
+// exclusive prefix of the items' tile counts; work[count] = total.  Wave w
+// takes items w span .. (w+1) span - 1 (span a multiple of 64) in rounds of
+// 64 consecutive items, so every load is coalesced, and scans each round
+// across its lanes on top of the running total
 __device__ void scan_items(const ClassDev& cd, int n, int* part) {
-    const int tid = threadIdx.x;
-    const int per = (n + kPlanThreads - 1) / kPlanThreads;
-    const int j0 = min(n, tid * per), j1 = min(n, j0 + per);
+    constexpr int NW = kPlanThreads / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int span = ((n + NW - 1) / NW + 63) & ~63;
+    const int j0 = min(n, wv * span), j1 = min(n, j0 + span);
     int sum = 0;
-    for (int j = j0; j < j1; ++j) sum += cd.items[j].tiles;
-    part[tid] = sum;
+    for (int j = j0 + lane; j < j1; j += 64) sum += cd.items[j].tiles;
+    for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane == 0) part[wv] = sum;
     __syncthreads();
-    for (int off = 1; off < kPlanThreads; off <<= 1) {   // inclusive scan
-        const int v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    int run = 0;
+    for (int i = 0; i < wv; ++i) run += part[i];
+    for (int j = j0; j < j1; j += 64) {
+        const int t = j + lane < j1 ? cd.items[j + lane].tiles : 0;
+        int incl = t;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (j + lane < j1) cd.work[j + lane] = run + incl - t;
+        run += __shfl(incl, 63);
     }
-    int run = part[tid] - sum;
-    for (int j = j0; j < j1; ++j) {
-        cd.work[j] = run;
-        run += cd.items[j].tiles;
+    if (tid == 0) {
+        int total = 0;
+        for (int i = 0; i < NW; ++i) total += part[i];
+        cd.work[n] = total;
     }
-    if (tid == kPlanThreads - 1) cd.work[n] = part[tid];
 }
 
 __global__ __launch_bounds__(kPlanThreads) void k_plan(const ClassDev* __restrict__ classes,
                                                        const UnitState* __restrict__ state,
                                                        const ScaleDev* __restrict__ scales, int full) {
-    __shared__ int part[kPlanThreads];
+    __shared__ int part[kPlanThreads / 64];   // per-wave totals of scan_items
     const ClassDev cd = classes[blockIdx.x];
     const int n = cd.njobs;
     for (int j = threadIdx.x; j < n; j += kPlanThreads) {
@@ -129,7 +139,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(const ClassDev* __restric
 
 // prefix over items appended by k_region_items (one workgroup per class)
 __global__ __launch_bounds__(kPlanThreads) void k_scan_items(const ClassDev* __restrict__ classes) {
-    __shared__ int part[kPlanThreads];
+    __shared__ int part[kPlanThreads / 64];   // per-wave totals of scan_items
     const ClassDev cd = classes[blockIdx.x];
     const int n = min(*cd.count, cd.max_items);
     scan_items(cd, n, part);
@@ -1096,7 +1106,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_certify(const UnitDev* __res
                                                             RegionState* __restrict__ regions, float delta,
                                                             unsigned long long* __restrict__ stats) {
     __shared__ float red[kRegionThreads];
-    __shared__ int viol;
+    __shared__ int nviol, ndirty;   // this unit's flagged and dirty regions
     const int u = blockIdx.x;
     if (!state[u].active) return;
     const UnitDev ud = units[u];
@@ -1108,7 +1118,7 @@ __global__ __launch_bounds__(kRegionThreads) void k_certify(const UnitDev* __res
         if (!(R->flags & 1)) best = fmaxf(best, R->rmax);
     }
     red[tid] = best;
-    if (tid == 0) viol = 0;
+    if (tid == 0) nviol = ndirty = 0;
     __syncthreads();
     for (int off = kRegionThreads / 2; off > 0; off >>= 1) {
         if (tid < off) red[tid] = fmaxf(red[tid], red[tid + off]);
@@ -1125,15 +1135,18 @@ __global__ __launch_bounds__(kRegionThreads) void k_certify(const UnitDev* __res
             ++mine;
         }
     }
-    if (mine) {
-        atomicAdd(&viol, 1);
-        atomicAdd(&stats[1], static_cast<unsigned long long>(mine));
-    }
-    atomicAdd(&stats[3], static_cast<unsigned long long>(dirty));
+    if (mine) atomicAdd(&nviol, mine);
+    if (dirty) atomicAdd(&ndirty, dirty);
     __syncthreads();
+    // one update of each statistic per unit (same-address atomics from every
+    // wave of every unit queue behind each other)
     if (tid == 0) {
         atomicAdd(&stats[2], 1ull);
-        if (viol) atomicAdd(&stats[0], 1ull);
+        if (nviol) {
+            atomicAdd(&stats[0], 1ull);
+            atomicAdd(&stats[1], static_cast<unsigned long long>(nviol));
+        }
+        if (ndirty) atomicAdd(&stats[3], static_cast<unsigned long long>(ndirty));
     }
 }
 
