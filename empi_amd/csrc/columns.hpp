@@ -24,19 +24,21 @@ constexpr int kCounterWords = 2 * kNumClasses * kCounterCopies;
 
 // Workgroup geometry of class Q: NSL slots of Q complex points in LDS (4096
 // points in at most 512 slots up to Q = 1024, two slots from Q = 2048 on),
-// 16 points per thread.  Shared by the kernels and the host's grid and tile
-// arithmetic.
+// 16 points per thread (32 at Q = 8192, whose 16,384 points would otherwise
+// take 1024 threads at 128 registers each).  Shared by the kernels and the
+// host's grid and tile arithmetic.
 constexpr int kClassPoints = 4096;
 constexpr int class_slots(int Q) {
     return Q >= 2048 ? 2 : (kClassPoints / Q > 512 ? 512 : kClassPoints / Q);
 }
-constexpr int class_threads(int Q) { return class_slots(Q) * Q / 16; }
+constexpr int class_points_per_thread(int Q) { return Q >= 8192 ? 32 : 16; }
+constexpr int class_threads(int Q) { return class_slots(Q) * Q / class_points_per_thread(Q); }
 
 template <int Q>
 struct Geo {
     static constexpr int NSL = class_slots(Q);
     static constexpr int NPTS = NSL * Q;
-    static constexpr int EPT = 16;
+    static constexpr int EPT = class_points_per_thread(Q);
     static constexpr int NT = NPTS / EPT;
     static constexpr int NCELL = NSL / 2;
 };
