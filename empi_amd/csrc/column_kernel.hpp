@@ -213,21 +213,22 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
     constexpr int NCW = WL ? NCELL / 4 : NCELL;
     static_assert(!WL || (NT == 256 && NCW >= 1 && NCW <= 64), "wave-local tiles");
     // Wave-local passes (Q = 8..4096, all modes): the Stockham passes run on
-    // slots of QP points that each wave holds whole.  Q <= 1024 passes over
-    // the Q-point slots themselves.  Q = 2048 and 4096 take the first step of
-    // a decimation in frequency at load time, in registers: with S = Q / 1024,
-    // the S inputs j + 1024 r (r < S) of a slot, which one thread loads, give
-    // y_s[j] = w_Q^{js} sum_r x[j + 1024 r] e^{-2 pi i rs/S}, and the slot's
-    // bin S k' + s is bin k' of the 1024-point transform of y_s.  A workgroup
+    // slots of QP points that each wave holds whole (a wave holds 64 EPT =
+    // 1024 points).  Smaller Q pass over the Q-point slots themselves.  Q =
+    // 2048 and 4096 take the first step of a decimation in frequency at load
+    // time, in registers: with S = Q / QP, the S inputs j + QP r (r < S) of a
+    // slot, which one thread loads, give
+    // y_s[j] = w_Q^{js} sum_r x[j + QP r] e^{-2 pi i rs/S}, and the slot's
+    // bin S k' + s is bin k' of the QP-point transform of y_s.  A workgroup
     // barrier then remains only after the load (where the tile is not
-    // wave-local) and before the products.  Q = 8192 (multichannel, 128
-    // registers per thread at 1024 threads) keeps workgroup passes: holding
-    // its sixteen decimation inputs spilled and ran 19% slower.
-    constexpr int QP = Q >= 2048 && Q <= 4096 ? 1024 : Q;
-    constexpr int S = Q / QP;
+    // wave-local) and before the products.  Q = 8192 (multichannel) keeps
+    // workgroup passes: the same step onto 2048-point slots spilled more
+    // registers there and ran 10% slower.
     constexpr bool WLP = Q >= 8 && Q <= 4096;
+    constexpr int QP = WLP && Q > 64 * EPT ? 64 * EPT : Q;
+    constexpr int S = Q / QP;
     static_assert(!WLP || (NPTS / (NT / 64)) % QP == 0, "wave-local passes");
-    static_assert(S == 1 || Q / NT == 8, "decimation step: eight pairs per thread, S of them 1024 apart");
+    static_assert(S == 1 || (Q / NT) % S == 0, "decimation step: a thread's pairs hold whole radix-S groups");
     __shared__ float2 buf[Lds<Q, NPTS>::SIZE];
     __shared__ CellInfo info[NCELL];
     __shared__ unsigned long long best[NCELL];
@@ -507,8 +508,8 @@ __global__ __launch_bounds__(Geo<Q>::NT) __attribute__((amdgpu_waves_per_eu(W)))
                         }
                     }
                 } else {
-                    // the thread's pairs j + 1024 r, r < S, for its GL / S values
-                    // of j (NT apart); pre-twiddles advance by w_M^{1024 k2} along r
+                    // the thread's pairs j + QP r, r < S, for its GL / S values
+                    // of j (NT apart); pre-twiddles advance by w_M^{QP k2} along r
                     constexpr int GS = GL / S;
                     const float2 sra = cis_rev(-static_cast<float>(__umul24(uka, static_cast<unsigned>(QP))) * invM);
                     const float2 srb = cis_rev(-static_cast<float>(__umul24(ukb, static_cast<unsigned>(QP))) * invM);
