@@ -49,15 +49,6 @@ struct Tw16 {
     static constexpr int SIZE = tw16_radix<Q>() > 0 ? 16 * tw16_radix<Q>() : 1;
 };
 
-// Orders one wave's LDS accesses across its lanes.  LDS serves a wave's
-// instructions in issue order, so only the compiler must be kept from moving
-// accesses across this point.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // One Stockham autosort pass of radix R at sub-length Ns over all slots of
 // Q points.  WL (wave-local): of NW = NT / 64 waves, wave w takes butterflies
 // w NB/NW .. (w+1) NB/NW - 1, i.e. slots w NSL/NW .. (w+1) NSL/NW - 1 in

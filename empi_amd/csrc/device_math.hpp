@@ -164,6 +164,15 @@ __device__ inline void twiddle_rev(float cyc, float* sn, float* cs) {
     *cs = __builtin_amdgcn_cosf(cyc);
 }
 
+// Orders one wave's LDS accesses across its lanes.  LDS serves a wave's
+// instructions in issue order, so only the compiler must be kept from moving
+// accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline unsigned long long shfl_xor_u64(unsigned long long v, int m) {
     const unsigned lo = __shfl_xor(static_cast<unsigned>(v), m);
     const unsigned hi = __shfl_xor(static_cast<unsigned>(v >> 32), m);

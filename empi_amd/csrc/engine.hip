@@ -773,36 +773,47 @@ __global__ __launch_bounds__(kRegionThreads) void k_prefix(const UnitDev* __rest
                                                            const float* __restrict__ res32,
                                                            double* __restrict__ prefix,
                                                            const long long* __restrict__ prefix_off) {
-    __shared__ double part[kRegionThreads];
+    // wave w takes samples w span .. (w+1) span - 1 in coalesced rounds of 64,
+    // scanned across the lanes on top of the running total
+    constexpr int NW = kRegionThreads / 64;
+    __shared__ double part[NW];
     const int u = blockIdx.x;
     if (!state[u].active) return;
     const UnitDev ud = units[u];
     const int N = ud.N;
     const float* x = res32 + ud.res_off;
     double* pre = prefix + prefix_off[u];
-    const int tid = threadIdx.x;
-    const int per = (N + kRegionThreads - 1) / kRegionThreads;
-    const int n0 = min(N, tid * per), n1 = min(N, n0 + per);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int span = ((N + NW - 1) / NW + 63) & ~63;
+    const int n0 = min(N, wv * span), n1 = min(N, n0 + span);
     double sum = 0.0;
-    for (int n = n0; n < n1; ++n) sum += static_cast<double>(x[n]) * x[n];
-    part[tid] = sum;
+    for (int n = n0 + lane; n < n1; n += 64) sum += static_cast<double>(x[n]) * x[n];
+    for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane == 0) part[wv] = sum;
     __syncthreads();
-    for (int off = 1; off < kRegionThreads; off <<= 1) {
-        const double v = tid >= off ? part[tid - off] : 0.0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    double run = 0.0;
+    for (int i = 0; i < wv; ++i) run += part[i];
+    for (int n = n0; n < n1; n += 64) {
+        const double e = n + lane < n1 ? static_cast<double>(x[n + lane]) * x[n + lane] : 0.0;
+        double incl = e;
+        for (int off = 1; off < 64; off <<= 1) {
+            const double t = __shfl_up(incl, off);
+            if (lane >= off) incl += t;
+        }
+        const double excl = __shfl_up(incl, 1);
+        if (n + lane < n1) pre[n + lane] = run + (lane ? excl : 0.0);
+        run += __shfl(incl, 63);
     }
-    double run = part[tid] - sum;
-    for (int n = n0; n < n1; ++n) {
-        pre[n] = run;
-        run += static_cast<double>(x[n]) * x[n];
+    if (tid == 0) {
+        double total = 0.0;
+        for (int i = 0; i < NW; ++i) total += part[i];
+        pre[N] = total;
     }
-    if (tid == kRegionThreads - 1) pre[N] = part[tid];
 }
 
 // log power spectra (64 pooled bins of a 256-point DFT) of the 256-sample
-// blocks the last atom touched, or of every block on init; one wave per block
+// blocks the last atom touched, or of every block on init; one wave per block,
+// which transforms it by four radix-4 Stockham passes in its own LDS buffer
 __global__ __launch_bounds__(kRegionThreads) void k_block_spectra(const UnitDev* __restrict__ units,
                                                                   const UnitState* __restrict__ state,
                                                                   const float* __restrict__ res32,
@@ -822,29 +833,43 @@ __global__ __launch_bounds__(kRegionThreads) void k_block_spectra(const UnitDev*
         b0 = max(0, static_cast<int>(lrint((st.tA - hw) * ud.fs)) / kBlock);
         b1 = min(ud.B - 1, static_cast<int>(lrint((st.tA + hw) * ud.fs)) / kBlock);
     }
-    // e^{-2 pi i j / 256}, the same hardware values per (k n) mod 256 as a
-    // per-product sine/cosine, looked up
+    // e^{-2 pi i j / 256} from the hardware sine/cosine
     __shared__ float2 wtab[kBlock];
+    constexpr int waves = kRegionThreads / 64;
+    __shared__ float2 wbuf[waves][kBlock + kBlock / 16];   // one float2 of padding per 16
     for (int j = threadIdx.x; j < kBlock; j += kRegionThreads)
         wtab[j] = cis_rev(-static_cast<float>(j) * (1.0f / kBlock));
     __syncthreads();
-    const int waves = kRegionThreads / 64;
     const int m = threadIdx.x & 63;
+    float2* z = wbuf[threadIdx.x >> 6];
+    auto pad = [](const int i) { return i + (i >> 4); };
     for (int b = b0 + blockIdx.x * waves + (threadIdx.x >> 6); b <= b1; b += gridDim.x * waves) {
-        float p = 0.0f;
-        for (int q = 1; q <= 2; ++q) {
-            const int k = 2 * m + q;
-            float re = 0.0f, im = 0.0f;
-            for (int n = 0; n < kBlock; ++n) {
-                const int i = b * kBlock + n;
-                const float v = i < N ? x[i] : 0.0f;
-                const float2 w = wtab[(k * n) & (kBlock - 1)];
-                re = fmaf(v, w.x, re);
-                im = fmaf(v, w.y, im);
-            }
-            p += re * re + im * im;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = b * kBlock + m + 64 * r;
+            z[pad(m + 64 * r)] = make_float2(i < N ? x[i] : 0.0f, 0.0f);
         }
-        bf[b * kFeatBins + m] = logf(1e-30f + p);
+        wave_sync();
+        // butterfly m of the pass at sub-length Ns: inputs m + 64 r, twiddles
+        // e^{-2 pi i r k / (4 Ns)}, k = m mod Ns, outputs (m - k) 4 + k + r Ns
+#pragma unroll
+        for (int Ns = 1; Ns < kBlock; Ns *= 4) {
+            const int k = m & (Ns - 1);
+            float2 v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = z[pad(m + 64 * r)];
+            wave_sync();
+#pragma unroll
+            for (int r = 1; r < 4; ++r) v[r] = cmul(v[r], wtab[(r * k * (kBlock / (4 * Ns))) & (kBlock - 1)]);
+            dft4(v);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) z[pad((m - k) * 4 + k + r * Ns)] = v[r];
+            wave_sync();
+        }
+        // bins 2m + 1 and 2m + 2
+        const float2 a = z[pad(2 * m + 1)], c = z[pad(2 * m + 2)];
+        bf[b * kFeatBins + m] = logf(1e-30f + (a.x * a.x + a.y * a.y + c.x * c.x + c.y * c.y));
+        wave_sync();   // the next block overwrites the buffer
     }
 }
 
