@@ -75,3 +75,21 @@ def test_mixed_variants_one_plan():
         res = plan.result(i)
         assert res.status == 0
         _check_item(res, oras[i], variant, f"mixed item {i} (variant {variant})")
+
+
+def test_full_build_counts_every_entry():
+    """A one-iteration run transforms the whole map once and nothing after it
+    (the last atom is not subtracted), so the entry counters, which the
+    column kernels keep in one copy per XCD and the host sums, must total
+    every unit's dictionary size: one per channel of an SMP item, one per
+    MMP3 item."""
+    _announce("entry counters: one iteration of a mixed SMP / MMP3 batch counts every map entry once")
+    rng = np.random.default_rng(77)
+    specs = [E.ItemSpec(1, 1280, 128.0, E.SMP, 1, 99.0, 0.01),
+             E.ItemSpec(3, 3000, 256.0, E.SMP, 1, 99.0, 0.01),
+             E.ItemSpec(1, 20000, 256.0, E.SMP, 1, 99.0, 0.01),
+             E.ItemSpec(4, 2000, 256.0, E.MMP3, 1, 99.0, 0.01)]
+    plan = _run_plan(specs, [rng.standard_normal((s.channels, s.samples)) for s in specs])
+    units = [s.channels if s.variant == E.SMP else 1 for s in specs]
+    expected = sum(u * plan.atom_count(i) for i, u in enumerate(units))
+    assert plan.entries_evaluated() == expected
