@@ -611,13 +611,20 @@ __device__ inline bool atom_range(const ScaleDev& sd, const UnitState& st, int* 
     return l <= h;
 }
 
-// region bests from the cell bests (and the low-bin bound seeds from the
-// cells' low-bin bests): all regions, or those flagged "evaluate"; one wave per region
+// region bests from the tile bests and the low-bin bound seeds from the
+// cells' low-bin bests: all regions, or those flagged "evaluate"; one wave
+// per region.  A region is whole tiles of its scale (its first column is
+// rounded to a tile boundary, and it ends where the next region starts or at
+// the scale's end), and a tile key orders its cells as the region does, by
+// cell best and then by the smallest cell, so the region's best is the best
+// of its tiles' bests compared by the cell best they carry
+// (pack_best(product, bin), recovered exactly from the key) and then by cell
 __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev* __restrict__ units,
                                                                    const UnitState* __restrict__ state,
                                                                    const ScaleDev* __restrict__ scales,
                                                                    const RegionDev* __restrict__ regtab,
-                                                                   const unsigned long long* __restrict__ cellbest,
+                                                                   const UnitScale* __restrict__ uscales,
+                                                                   const unsigned long long* __restrict__ tilebest,
                                                                    const float* __restrict__ celllow,
                                                                    RegionState* __restrict__ regions, int all) {
     const int u = blockIdx.y;
@@ -638,14 +645,20 @@ __global__ __launch_bounds__(kRegionThreads) void k_regions_reduce(const UnitDev
         if (rt.tlo <= rt.thi) {
             const long long base = ud.cell_off + sd.cells_before;
             const int c0 = rt.tlo * sd.G, c1 = rt.thi * sd.G + sd.G - 1;
-            for (int c = c0 + lane; c <= c1; c += 64) {
-                const unsigned long long v = cellbest[base + c];
+            const UnitScale us = uscales[ud.us_off + l];
+            const int ncell = us.ncell;
+            for (int t = c0 / ncell + lane; t <= c1 / ncell; t += 64) {
+                const unsigned long long k = tilebest[us.tile_off + t];
+                const unsigned key = 0xFFFFFFFFu - static_cast<unsigned>(k);
+                const unsigned bin = key >> kTileCellBits;
+                const int c = t * ncell + static_cast<int>(key & ((1u << kTileCellBits) - 1u));
+                const unsigned long long v = (k & 0xFFFFFFFF00000000ull) | (0xFFFFFFFFu - bin);
                 if (v > best || (v == best && c < bcell)) {
                     best = v;
                     bcell = c;
                 }
-                low = fmaxf(low, celllow[base + c]);
             }
+            for (int c = c0 + lane; c <= c1; c += 64) low = fmaxf(low, celllow[base + c]);
         }
         for (int off = 32; off >= 1; off >>= 1) {
             const unsigned long long ov = shfl_xor_u64(best, off);
@@ -1982,7 +1995,7 @@ empi_status proposal_pass(empi_plan* p, hipStream_t s, unsigned long long* stat)
     empi_status st = launch_class_columns(p, s);
     if (st != EMPI_OK) return st;
     hipLaunchKernelGGL(k_regions_reduce, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p, p->d_state.p,
-                       p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_celllow.p, p->d_regions.p, 0);
+                       p->d_scales.p, p->d_regtab.p, p->d_uscales.p, p->d_tiles.p, p->d_celllow.p, p->d_regions.p, 0);
     EMPI_HIP(hipGetLastError());
     return EMPI_OK;
 }
@@ -2185,7 +2198,8 @@ empi_status plan_run(empi_plan* p, const float* signal, int on_device, hipStream
     if (prop) {
         EMPI_HIP(hipMemsetAsync(p->d_pstats.p, 0, sizeof(unsigned long long) * 5, s));
         hipLaunchKernelGGL(k_regions_reduce, dim3(kSlices, U), dim3(kRegionThreads), 0, s, p->d_units.p,
-                           p->d_state.p, p->d_scales.p, p->d_regtab.p, p->d_cells.p, p->d_celllow.p, p->d_regions.p, 1);
+                           p->d_state.p, p->d_scales.p, p->d_regtab.p, p->d_uscales.p, p->d_tiles.p, p->d_celllow.p,
+                           p->d_regions.p, 1);
         EMPI_HIP(hipGetLastError());
         if ((st = launch_features(p, s, 1)) != EMPI_OK) return st;
         if (graph && !p->prop_exec) {
