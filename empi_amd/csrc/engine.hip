@@ -934,80 +934,119 @@ __global__ __launch_bounds__(kRegionThreads) void k_rows(
     const float lref = logf(static_cast<float>(ref));
     const float eref = static_cast<float>(st.totalEnergy * ud.fs) * st.xscale * st.xscale;
     const float limit = pbest[u] / (1.0f - delta);
-    for (int r = blockIdx.x * kRegionThreads + threadIdx.x; r < ud.L * ud.B; r += gridDim.x * kRegionThreads) {
-        RegionState* R = &regions[ud.reg_off + r];
-        if (!(R->flags & 1)) continue;
-        const int l = r / ud.B, b = r % ud.B;
+    // Candidate rows are taken a wave at a time (a ballot and one atomic), so a
+    // wave's candidates own consecutive rows in lane order; the wave then
+    // stages them 16 at a time in LDS (the 64 block features by one coalesced
+    // load per row, the 16 scalar features from their owners) and writes the
+    // staged rows as one contiguous block, instead of each lane writing its
+    // own 320-byte row across 80 scattered stores.
+    constexpr int NW = kRegionThreads / 64;
+    static_assert(kFeatBins == 64 && kFeatDim == kFeatBins + 16, "a row: one feature per lane, then 16 scalars");
+    __shared__ float stage[NW][16][kFeatDim + 1];
+    __shared__ int stage_b[NW][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nreg = ud.L * ud.B;
+    for (int r0 = blockIdx.x * kRegionThreads; r0 < nreg; r0 += gridDim.x * kRegionThreads) {   // uniform rounds
+        const int r = r0 + static_cast<int>(threadIdx.x);
+        const int rc = min(r, nreg - 1);   // records of the round's last region for threads past it
+        RegionState* R = &regions[ud.reg_off + rc];
+        const int l = rc / ud.B, b = rc % ud.B;
         const ScaleDev& sd = scales[ud.scale0 + l];
-        const RegionDev rt = regtab[ud.regtab_off + r];
-        // bins j >= JB and bins 0, M: (B) at E <= 1e-4 or a real value;
-        // bins 1 <= j < JB: the smaller of (A) from their |u| seeds and (B)
-        const float hi = sqrtf(R->rmax) + R->racc * sd.gH;
-        const float loA = (sqrtf(R->rlow) + R->racc_lo) * sd.gL;
-        const float loB = sqrtf(R->rmax) + R->racc_lo * sd.lL;
-        const float lo = fminf(loA, loB);
-        const float grow = fmaxf(hi * hi, lo * lo) * 1.0001f;
-        // Cauchy-Schwarz per column: P(t, f) <= csnorm * (residual energy under
-        // column t's window [iL, iR]), at the region's largest.  A region of
-        // many narrow columns spans far more samples than any one window.
-        // Only needed where the growth bound fails (the certification's limit
-        // is never below this one).  Where the columns are many (more than 16
-        // and more than the starts below), windows of Wmax + S samples at
-        // starts S apart (S = Wmax / 2) cover every column's window
-        // [iL, iL + Wmax) between the first and the last start.
-        float bound = grow;
-        if (grow > limit || all_rows) {
-            const double hw = kGaussHalfWidth * sd.s;
-            auto win = [&](int t, int* iL, int* iR) {
-                const double tc = t * sd.tMax / (sd.T - 1);
-                const long wl = lrint((tc - hw) * sd.fs), wr = lrint((tc + hw) * sd.fs);
-                *iL = static_cast<int>(wl > 0 ? wl : 0);
-                *iR = static_cast<int>(wr < N - 1 ? wr : N - 1);
-            };
-            int il0, il1, ir0, ir1;
-            win(rt.tlo, &il0, &ir0);
-            win(rt.thi, &il1, &ir1);
-            const int S = max(1, sd.Wmax / 2);
-            double emax = 0.0;
-            const int ncol = rt.thi - rt.tlo + 1;
-            if (ncol > 16 && (il1 - il0) / S + 1 < ncol) {
+        bool want = false;
+        float bound = 0.0f;
+        if (r < nreg && (R->flags & 1)) {
+            const RegionDev rt = regtab[ud.regtab_off + r];
+            // bins j >= JB and bins 0, M: (B) at E <= 1e-4 or a real value;
+            // bins 1 <= j < JB: the smaller of (A) from their |u| seeds and (B)
+            const float hi = sqrtf(R->rmax) + R->racc * sd.gH;
+            const float loA = (sqrtf(R->rlow) + R->racc_lo) * sd.gL;
+            const float loB = sqrtf(R->rmax) + R->racc_lo * sd.lL;
+            const float lo = fminf(loA, loB);
+            const float grow = fmaxf(hi * hi, lo * lo) * 1.0001f;
+            // Cauchy-Schwarz per column: P(t, f) <= csnorm * (residual energy under
+            // column t's window [iL, iR]), at the region's largest.  A region of
+            // many narrow columns spans far more samples than any one window.
+            // Only needed where the growth bound fails (the certification's limit
+            // is never below this one).  Where the columns are many (more than 16
+            // and more than the starts below), windows of Wmax + S samples at
+            // starts S apart (S = Wmax / 2) cover every column's window
+            // [iL, iL + Wmax) between the first and the last start.
+            bound = grow;
+            if (grow > limit || all_rows) {
+                const double hw = kGaussHalfWidth * sd.s;
+                auto win = [&](int t, int* iL, int* iR) {
+                    const double tc = t * sd.tMax / (sd.T - 1);
+                    const long wl = lrint((tc - hw) * sd.fs), wr = lrint((tc + hw) * sd.fs);
+                    *iL = static_cast<int>(wl > 0 ? wl : 0);
+                    *iR = static_cast<int>(wr < N - 1 ? wr : N - 1);
+                };
+                int il0, il1, ir0, ir1;
+                win(rt.tlo, &il0, &ir0);
+                win(rt.thi, &il1, &ir1);
+                const int S = max(1, sd.Wmax / 2);
+                double emax = 0.0;
+                const int ncol = rt.thi - rt.tlo + 1;
+                if (ncol > 16 && (il1 - il0) / S + 1 < ncol) {
 #pragma unroll 4
-                for (int i = il0; i <= il1; i += S) emax = fmax(emax, pre[min(i + sd.Wmax + S, N)] - pre[i]);
-                emax = fmin(emax, pre[ir1 + 1] - pre[il0]);   // nor above the union of the windows
-            } else {
+                    for (int i = il0; i <= il1; i += S) emax = fmax(emax, pre[min(i + sd.Wmax + S, N)] - pre[i]);
+                    emax = fmin(emax, pre[ir1 + 1] - pre[il0]);   // nor above the union of the windows
+                } else {
 #pragma unroll 4
-                for (int t = rt.tlo; t <= rt.thi; ++t) {
-                    int iL, iR;
-                    win(t, &iL, &iR);
-                    emax = fmax(emax, pre[iR + 1] - pre[iL]);
+                    for (int t = rt.tlo; t <= rt.thi; ++t) {
+                        int iL, iR;
+                        win(t, &iL, &iR);
+                        emax = fmax(emax, pre[iR + 1] - pre[iL]);
+                    }
                 }
+                bound = fminf(sd.csnorm * static_cast<float>(emax), grow);
             }
-            bound = fminf(sd.csnorm * static_cast<float>(emax), grow);
+            R->rbound = bound;
+            const bool candidate = bound > limit;
+            R->flags = candidate ? (R->flags | 4) : (R->flags & ~4);
+            want = candidate || all_rows;
         }
-        R->rbound = bound;
-        const bool candidate = bound > limit;
-        R->flags = candidate ? (R->flags | 4) : (R->flags & ~4);
-        if (!candidate && !all_rows) continue;
-        const int row = atomicAdd(rowcount, 1);
-        if (row >= max_rows) continue;
-        float* f = rows + static_cast<long long>(row) * kFeatDim;
-        for (int q = 0; q < kFeatBins; ++q) f[q] = bf[b * kFeatBins + q] - lref;
-        f[64] = logf(1e-30f + bound / static_cast<float>(ref));
-        f[65] = logf(1e-30f + R->rmax / static_cast<float>(ref));
-        const float ls = log2f(sd.s * sd.fs) * (1.0f / 16.0f);
-        f[66] = ls;
-        f[67] = ls * ls;
-        f[68] = __builtin_amdgcn_sinf(0.5f * ls);
-        f[69] = __builtin_amdgcn_cosf(0.5f * ls);
-        f[70] = __builtin_amdgcn_sinf(ls);
-        f[71] = __builtin_amdgcn_cosf(ls);
-        f[72] = __builtin_amdgcn_sinf(2.0f * ls);
-        f[73] = __builtin_amdgcn_cosf(2.0f * ls);
-        f[74] = static_cast<float>(min(kBlock, N - b * kBlock)) / kBlock;
-        f[75] = logf(1e-30f + static_cast<float>(ref) / (eref > 0.0f ? eref : 1.0f));
-        f[76] = f[77] = f[78] = f[79] = 0.0f;
-        rowmap[row] = static_cast<int>(ud.reg_off + r);
-        rowref[row] = static_cast<float>(ref);
+        const unsigned long long m = __ballot(want);
+        if (m == 0ull) continue;   // uniform over the wave
+        const int n = __popcll(m);
+        const int lead = __ffsll(static_cast<long long>(m)) - 1;
+        int base = 0;
+        if (lane == lead) base = atomicAdd(rowcount, n);
+        base = __shfl(base, lead);
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        const int row = base + rank;
+        if (want && row < max_rows) {
+            rowmap[row] = static_cast<int>(ud.reg_off + r);
+            rowref[row] = static_cast<float>(ref);
+        }
+        for (int g = 0; g < n; g += 16) {
+            const int cnt = min(16, n - g);
+            if (want && rank >= g && rank < g + cnt) {
+                float* sc = stage[wv][rank - g] + kFeatBins;
+                stage_b[wv][rank - g] = b;
+                sc[0] = logf(1e-30f + bound / static_cast<float>(ref));
+                sc[1] = logf(1e-30f + R->rmax / static_cast<float>(ref));
+                const float ls = log2f(sd.s * sd.fs) * (1.0f / 16.0f);
+                sc[2] = ls;
+                sc[3] = ls * ls;
+                sc[4] = __builtin_amdgcn_sinf(0.5f * ls);
+                sc[5] = __builtin_amdgcn_cosf(0.5f * ls);
+                sc[6] = __builtin_amdgcn_sinf(ls);
+                sc[7] = __builtin_amdgcn_cosf(ls);
+                sc[8] = __builtin_amdgcn_sinf(2.0f * ls);
+                sc[9] = __builtin_amdgcn_cosf(2.0f * ls);
+                sc[10] = static_cast<float>(min(kBlock, N - b * kBlock)) / kBlock;
+                sc[11] = logf(1e-30f + static_cast<float>(ref) / (eref > 0.0f ? eref : 1.0f));
+                sc[12] = sc[13] = sc[14] = sc[15] = 0.0f;
+            }
+            wave_sync();
+            for (int i = 0; i < cnt; ++i) stage[wv][i][lane] = bf[stage_b[wv][i] * kFeatBins + lane] - lref;
+            wave_sync();
+            // rows base + g .. base + g + cnt - 1 (those below max_rows) are one block
+            const long long first = static_cast<long long>(base) + g;
+            const int keep = static_cast<int>(min(static_cast<long long>(cnt), max(0ll, max_rows - first)));
+            for (int j = lane; j < keep * kFeatDim; j += 64) rows[first * kFeatDim + j] = stage[wv][j / kFeatDim][j % kFeatDim];
+            wave_sync();
+        }
     }
 }
 
