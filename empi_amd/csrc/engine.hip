@@ -1800,6 +1800,14 @@ empi_status plan_build(empi_plan* p, const empi_item* items, int count, int ws_c
                 r.tlo = static_cast<int>(std::min<long long>(first_t(b), T));
                 r.thi = static_cast<int>(std::min<long long>(first_t(b + 1) - 1, T - 1));
                 if (b == B - 1) r.thi = static_cast<int>(T - 1);
+                // the region reduction takes a region's best from its tiles'
+                // bests, so a region with columns must be whole tiles (up to
+                // the scale's end)
+                if (r.tlo <= r.thi) {
+                    const long long c0 = static_cast<long long>(r.tlo) * sc.G;
+                    const long long c1 = (static_cast<long long>(r.thi) + 1) * sc.G;
+                    if (c0 % cpw != 0 || (c1 % cpw != 0 && r.thi != T - 1)) return EMPI_INTERNAL;
+                }
                 p->regtab_h.push_back(r);
             }
         }
